@@ -1,0 +1,125 @@
+/*
+ * oracle.h -- the oracle for flow's hydro timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a CPU restatement, in plain C + OpenMP, of
+ * UoB-HPC/flow's omp3 kernel set (omp3/flow.c) and of the UoB-HPC/arch
+ * behaviours it calls (reflective boundary, minmod, uniform mesh geometry,
+ * problem rasterisation). Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker or the timed CPU
+ * baseline. The product (flow_amd/) never links or calls it.
+ *
+ * Arrays use the reference layouts: with L = nx (local size incl. pad),
+ *   cell   C: index i*L + j,      L rows
+ *   x-face X: index i*(L+1) + j,  L rows
+ *   y-face Y: index i*L + j,      L+1 rows
+ *   corner K: index i*(L+1) + j,  L+1 rows
+ * and every array is allocated (L+1)*(L+1) doubles (flow_data.c:8-15).
+ *
+ * Build: gcc -O3 -fopenmp -ffp-contract=off (no FMA contraction, no fast-math),
+ * so that every expression rounds exactly as written.
+ */
+#ifndef FLOW_ORACLE_H
+#define FLOW_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary inversion flags (arch shared.h convention). */
+#define ORC_NO_INVERT 0
+#define ORC_INVERT_X 1
+#define ORC_INVERT_Y 2
+
+typedef struct {
+  int nx, ny;               /* local sizes including 2*pad (main.c:24-25) */
+  int pad;                  /* 2 (main.c:23) */
+  int global_nx, global_ny; /* interior sizes */
+  double width, height;     /* domain extent (arch.params) */
+  double max_dt, sim_end;   /* arch.params */
+  double dt, dt_h;          /* carried timestep pair (main.c:30-31) */
+  double *edgex, *edgey;    /* left/bottom edge coordinate of each cell, L+1 */
+  double *edgedx, *edgedy;  /* face spacing, L+1 */
+  double *celldx, *celldy;  /* cell spacing, L+1 */
+  int verbose;              /* print the reference's per-step lines */
+} OrcMesh;
+
+/* ---- arch layer (restated) ---- */
+double orc_minmod(double a, double b);
+double orc_min(double a, double b);
+void orc_mesh_init(OrcMesh* m, int global_nx, int global_ny, double width,
+                   double height, double max_dt, double sim_end,
+                   double dt_param);
+void orc_mesh_free(OrcMesh* m);
+void orc_handle_boundary_2d(int nx, int ny, int pad, double* arr, int invert);
+/* Apply one problem box: cells whose left/bottom edge lies in
+ * [pos, pos+extent) (fractions of the domain) get density/energy. */
+void orc_rasterise_box(const OrcMesh* m, double* density, double* energy,
+                       double rho, double e, double xpos, double ypos,
+                       double w, double h);
+
+/* ---- omp3 stages (restated; each cites omp3/flow.c) ---- */
+void orc_equation_of_state(int nx, int ny, double* P, const double* rho,
+                           const double* e);
+void orc_pressure_acceleration(int nx, int ny, const OrcMesh* m, double dt,
+                               double* mom_x, double* mom_y, double* u,
+                               double* v, const double* P, const double* rho);
+void orc_artificial_viscosity(int nx, int ny, const OrcMesh* m, double dt,
+                              double* Qxx, double* Qyy, double* u, double* v,
+                              double* mom_x, double* mom_y, const double* rho);
+void orc_shock_heating_and_work(int nx, int ny, const OrcMesh* m, double dt_h,
+                                double* e, const double* P, const double* u,
+                                const double* v, const double* rho,
+                                const double* Qxx, const double* Qyy);
+/* Returns the local min dt before the clamp, for tests. */
+double orc_set_timestep(int nx, int ny, const double* Qxx, const double* Qyy,
+                        const double* rho, const double* e, OrcMesh* m,
+                        int first_step);
+void orc_x_mass_and_energy_flux(int nx, int ny, int first, const OrcMesh* m,
+                                double dt, double dt_h, double* rho,
+                                double* rho_old, double* e, const double* u,
+                                double* F_x, double* eF_x);
+void orc_y_mass_and_energy_flux(int nx, int ny, int first, const OrcMesh* m,
+                                double dt, double dt_h, double* rho,
+                                double* rho_old, double* e, const double* v,
+                                double* F_y, double* eF_y);
+void orc_advect_mass_and_energy(int nx, int ny, const OrcMesh* m, int tt,
+                                double dt, double dt_h, double* rho, double* e,
+                                double* rho_old, double* F_x, double* F_y,
+                                double* eF_x, double* eF_y, const double* u,
+                                const double* v);
+void orc_momentum_x_flux_in_x(int nx, int ny, const OrcMesh* m, double dt_h,
+                              const double* u, double* uF_x, const double* F_x);
+void orc_momentum_x_flux_in_y(int nx, int ny, const OrcMesh* m, double dt_h,
+                              const double* u, const double* v, double* uF_y,
+                              const double* F_y);
+void orc_momentum_y_flux_in_x(int nx, int ny, const OrcMesh* m, double dt_h,
+                              const double* u, const double* v, double* vF_x,
+                              const double* F_x);
+void orc_momentum_y_flux_in_y(int nx, int ny, const OrcMesh* m, double dt_h,
+                              const double* v, double* vF_y, const double* F_y);
+void orc_advect_momentum(int nx, int ny, int tt, const OrcMesh* m, double dt_h,
+                         double dt, double* u, double* v, double* uF_x,
+                         double* uF_y, double* vF_x, double* vF_y,
+                         double* mom_x, double* mom_y, const double* rho,
+                         const double* F_x, const double* F_y);
+void orc_conservation(int nx, int ny, int pad, const double* rho,
+                      const double* e, double* mass, double* energy);
+void orc_solve_hydro_2d(OrcMesh* m, int tt, double* P, double* rho,
+                        double* rho_old, double* e, double* u, double* v,
+                        double* mom_x, double* mom_y, double* Qxx, double* Qyy,
+                        double* F_x, double* F_y, double* uF_x, double* uF_y,
+                        double* vF_x, double* vF_y);
+
+/* The main.c loop (main.c:64-115) without I/O: pre-loop set_timestep with
+ * first_step=0, then niters x (solve + conservation), stopping at sim_end.
+ * Records per-step dt (the value printed at the start of each step), mass and
+ * energy (rho*e) totals. Returns the number of steps taken.
+ * `arrays` holds the 16 state/scratch arrays in solve_hydro_2d order:
+ * P rho rho_old e u v mom_x mom_y Qxx Qyy F_x F_y uF_x uF_y vF_x vF_y. */
+int orc_run(OrcMesh* m, double** arrays, int niters, double* dt_seq,
+            double* mass_seq, double* energy_seq, double* wall_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
