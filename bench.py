@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the hip kernel set on the multi-blast workload.
+
+One step = one solve_hydro_2d + print_conservation pass (main.c:82-91) over an
+8192^2 FP64 mesh (4x4 seeded blasts, log-uniform pressures in [10, 1000] on a
+rho=1 p=0.1 background, +/-5% seeded density noise). Inputs are resident in
+HBM before the timed region; the driver loop runs on the host through the C
+ABI (flowhip_run), with one device-to-host transfer of the scalars per
+interface call, exactly as main.c would drive it.
+
+Prints ONE JSON line: whole-job cell-updates/s, the roofline of the dominant
+kernel (HIP-event durations on the kernel set's stream x its algorithmic
+bytes), the full-step fraction of HBM peak at B_alg = 376 B per cell-update,
+and the oracle (CPU restatement of omp3) timed on this host's cores on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s and % HBM peak at 8192^2, FP64 vs FP32 state; "
+          "ensemble solves/s")
+HBM_PEAK = 8.0e12            # MI355X HBM3E, B/s
+B_ALG_FP64 = 376             # bytes per cell-update: 47 compulsory field sweeps
+B_ALG_FP32 = 188
+
+# Algorithmic bytes per launch, in field sweeps of 8 B over the interior
+# (each distinct array a kernel reads counted once, each it writes once).
+KERNEL_SWEEPS = {
+    # stagewise schedule (one kernel per omp3 loop)
+    "eos": 3, "pressure_accel": 8, "av_stress": 5, "av_accel": 9,
+    "shock_heating": 8, "timestep": 4, "copy_rho_old": 2,
+    "x_flux": 5, "y_flux": 5, "x_update": 7, "y_update": 7,
+    "mom_flux_xx": 3, "mom_flux_xy": 4, "mom_flux_yx": 4, "mom_flux_yy": 3,
+    "mom_x_update_u": 5, "mom_y_update_v": 5, "mom_x_update": 3,
+    "mom_y_update": 3, "conservation": 2,
+    # fused schedule
+    "lagrangian": 9, "advect_mass_energy": 8, "advect_momentum_x": 8,
+    "advect_momentum_y": 8, "advect_momentum": 11,
+}
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=8192)
+    p.add_argument("--dt", type=float, default=1.0e-6,
+                   help=".params dt (the first step uses C_T * dt)")
+    p.add_argument("--seed", type=int, default=20250101)
+    p.add_argument("--schedule", choices=("fused", "stagewise"), default=None)
+    p.add_argument("--graphs", action="store_true")
+    p.add_argument("--timing-steps", type=int, default=20)
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="approximate CPU time budget of the oracle sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def reset_state(st, rho0, e0, mesh, dt_param):
+    import flow_amd
+    for k in flow_amd.ARRAY_NAMES:
+        st.zero(k)
+    st.upload("rho", rho0)
+    st.upload("e", e0)
+    st.reflect_initial()
+    mesh.set_dt(0.3 * dt_param, 0.3 * dt_param)
+
+
+def traffic_from_profiles(kernel, n):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE (scaled by a copy-kernel calibration), if committed."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^traffic_bytes_per_launch\s+(\S+)\s+(\d+)\s+([0-9.eE+]+)")
+    for line in open(path):
+        m = pat.match(line.strip())
+        if m and m.group(1) == kernel and int(m.group(2)) == n:
+            return float(m.group(3))
+    return None
+
+
+def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
+    """The oracle on this host's cores, from the same initial state, for as
+    many steps as fit in about budget_s seconds (at least one)."""
+    from oracle import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = po.Mesh(n, n, dt=dt_param)
+    a = po.new_arrays(m)
+    a["rho"][:] = rho0
+    a["e"][:] = e0
+    po.reflect_initial(m, a)
+    po.set_timestep(m, a, 0)
+    wall, steps = 0.0, 0
+    while True:
+        t0 = time.perf_counter()
+        po.solve_step(m, a, steps)
+        po.conservation(m, a)
+        wall += time.perf_counter() - t0
+        steps += 1
+        if wall >= budget_s or wall / steps * (steps + 1) > 2.0 * budget_s:
+            break
+    return {
+        "value": n * n * steps / wall, "unit": "cell-updates/s",
+        "cores": threads, "kind": "port",
+        "sample": f"oracle (C+OpenMP restatement of omp3/flow.c) on the "
+                  f"{n}x{n} multi-blast, first {steps} steps "
+                  f"({wall:.1f} s, {threads} threads)",
+        "steps": steps, "wall_s": wall,
+    }, a, steps
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (the multi-GPU row-slab design is documented, not "
+              "built)", file=sys.stderr)
+        sys.exit(2)
+    import flow_amd
+    from flow_amd import native, problems
+
+    n = args.n
+    rho0, e0 = problems.multi_blast(n, seed=args.seed)
+    mesh = flow_amd.Mesh(n, n, dt=args.dt)
+    fused_ok = hasattr(native.lib(), "fk_lagrangian")
+    sched_name = args.schedule or ("fused" if fused_ok else "stagewise")
+    mesh.set_schedule(flow_amd.FUSED if sched_name == "fused" else flow_amd.STAGEWISE)
+    mesh.set_graphs(args.graphs)
+    st = flow_amd.DeviceState(mesh)
+
+    # warm-up, then the timed run from the initial state
+    reset_state(st, rho0, e0, mesh, args.dt)
+    if args.warmup:
+        st.run(args.warmup)
+    reset_state(st, rho0, e0, mesh, args.dt)
+    st.set_timestep(0)
+    native.sync()
+    t0 = time.perf_counter()
+    steps, dts, mass, energy, _ = st.run(args.steps, pre_timestep=False)
+    native.sync()
+    t1 = time.perf_counter()
+    if steps != args.steps or not np.all(np.isfinite(dts)):
+        raise RuntimeError(f"run stopped after {steps} steps")
+    elapsed = t1 - t0
+    cups = n * n * steps / elapsed
+
+    # per-kernel HIP-event timing over the same workload
+    reset_state(st, rho0, e0, mesh, args.dt)
+    st.set_timestep(0)
+    mesh.reset_timing()
+    mesh.set_timing(True)
+    st.run(args.timing_steps, pre_timestep=False)
+    mesh.set_timing(False)
+    timers = mesh.timing()
+    dom = max((k for k in timers if k in KERNEL_SWEEPS),
+              key=lambda k: timers[k][0])
+    dom_ms, dom_launches = timers[dom]
+    avg_s = dom_ms / dom_launches * 1e-3
+    alg_bytes = KERNEL_SWEEPS[dom] * 8.0 * n * n
+    achieved = alg_bytes / avg_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": traffic_from_profiles(dom, n),
+                "kernel": dom, "avg_launch_ms": dom_ms / dom_launches,
+                "alg_bytes_per_launch": alg_bytes}
+    kernels = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / args.timing_steps}
+               for k, v in sorted(timers.items(), key=lambda kv: -kv[1][0])}
+
+    result = {
+        "metric": METRIC, "value": cups, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"multi-blast {n}x{n} FP64 (4x4 seeded blasts, "
+                               f"p log-U[10,1000], rho noise 5%, seed {args.seed}), "
+                               f"{steps} steps",
+                   "cells": n * n, "schedule": sched_name, "graphs": args.graphs,
+                   "dt_param": args.dt},
+        "roofline": roofline,
+        "step_roofline": {"b_alg_per_cell_update": B_ALG_FP64,
+                          "achieved_GBps": cups * B_ALG_FP64 / 1e9,
+                          "frac": cups * B_ALG_FP64 / HBM_PEAK},
+        "kernels": kernels,
+        "device": native.device_name(),
+    }
+
+    if not args.no_cpu_baseline:
+        base, ora, k = cpu_baseline(n, args.seed, args.dt, args.cpu_seconds,
+                                    rho0, e0)
+        result["cpu_baseline"] = {kk: base[kk] for kk in
+                                  ("value", "unit", "cores", "kind", "sample")}
+        # parity at full size on the same sample: k GPU steps vs k oracle steps
+        reset_state(st, rho0, e0, mesh, args.dt)
+        got_steps = st.run(k)[0]
+        got = st.download_all()
+        from tests import harness
+        cmp = harness.compare_state(got, ora, n + 4)
+        result["parity_vs_oracle"] = {
+            "steps": got_steps,
+            "max_rel_linf": max(v[0] for v in cmp.values()),
+            "bitwise_diffs": sum(v[1] for v in cmp.values())}
+    else:
+        result["cpu_baseline"] = None
+
+    st.close()
+    mesh.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * ctx.c -- creation of the hip kernel set's per-Mesh state and its
+ * per-kernel event timers.
+ */
+#include <string.h>
+
+#include "flowhip_ctx.h"
+
+FlowHipCtx* flowhip_ctx(Mesh* mesh) {
+  if (mesh->hip) return (FlowHipCtx*)mesh->hip;
+  FlowHipCtx* h = (FlowHipCtx*)calloc(1, sizeof(FlowHipCtx));
+  if (!h) TERMINATE("hip kernel set: out of host memory");
+  HIPCHECK(hipGetDevice(&h->device));
+  /* default priority: the kernel set never uses prioritised streams */
+  HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const int np = fk_reduce_blocks();
+  HIPCHECK(hipMalloc((void**)&h->red.partials, sizeof(double) * np));
+  HIPCHECK(hipMalloc((void**)&h->red.ticket, sizeof(unsigned int) * 4));
+  HIPCHECK(hipMalloc((void**)&h->red.dtp, sizeof(double) * 2));
+  HIPCHECK(hipMalloc((void**)&h->red.sums, sizeof(double) * 4));
+  HIPCHECK(hipMalloc((void**)&h->red.status, sizeof(int) * 4));
+  HIPCHECK(hipMemset(h->red.partials, 0, sizeof(double) * np));
+  HIPCHECK(hipMemset(h->red.ticket, 0, sizeof(unsigned int) * 4));
+  HIPCHECK(hipMemset(h->red.dtp, 0, sizeof(double) * 2));
+  HIPCHECK(hipMemset(h->red.sums, 0, sizeof(double) * 4));
+  HIPCHECK(hipMemset(h->red.status, 0, sizeof(int) * 4));
+  h->red.max_blocks = np / 4;
+  HIPCHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 8,
+                         hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc((void**)&h->h_status, sizeof(int) * 4,
+                         hipHostMallocDefault));
+  memset(h->h_stage, 0, sizeof(double) * 8);
+  memset(h->h_status, 0, sizeof(int) * 4);
+  h->schedule = FLOWHIP_STAGEWISE;
+  HIPCHECK(hipDeviceSynchronize());
+  mesh->hip = h;
+  return h;
+}
+
+void flowhip_ctx_destroy(Mesh* mesh) {
+  FlowHipCtx* h = (FlowHipCtx*)mesh->hip;
+  if (!h) return;
+  hipStreamSynchronize(h->stream);
+  for (int g = 0; g < 3; ++g)
+    if (h->graph[g]) hipGraphExecDestroy(h->graph[g]);
+  if (h->events_created) {
+    for (int k = 0; k < FLOWHIP_MAX_PENDING; ++k) {
+      hipEventDestroy(h->ev_start[k]);
+      hipEventDestroy(h->ev_stop[k]);
+    }
+  }
+  hipFree(h->red.partials);
+  hipFree(h->red.ticket);
+  hipFree(h->red.dtp);
+  hipFree(h->red.sums);
+  hipFree(h->red.status);
+  hipHostFree(h->h_stage);
+  hipHostFree(h->h_status);
+  hipStreamDestroy(h->stream);
+  free(h);
+  mesh->hip = NULL;
+}
+
+FkMesh flowhip_fkmesh(const Mesh* mesh) {
+  FkMesh m;
+  m.nx = mesh->local_nx;
+  m.ny = mesh->local_ny;
+  m.pad = mesh->pad;
+  m.edgedx = mesh->edgedx;
+  m.edgedy = mesh->edgedy;
+  m.celldx = mesh->celldx;
+  m.celldy = mesh->celldy;
+  return m;
+}
+
+void flowhip_invalidate_sums(Mesh* mesh) {
+  if (mesh && mesh->hip) ((FlowHipCtx*)mesh->hip)->sums_valid = 0;
+}
+
+static int timer_index(FlowHipCtx* h, const char* name) {
+  for (int k = 0; k < h->ntimers; ++k)
+    if (h->timers[k].name == name || strcmp(h->timers[k].name, name) == 0)
+      return k;
+  if (h->ntimers == FLOWHIP_MAX_TIMERS) return -1;
+  h->timers[h->ntimers].name = name;
+  h->timers[h->ntimers].total_ms = 0.0;
+  h->timers[h->ntimers].launches = 0;
+  return h->ntimers++;
+}
+
+int flowhip_timer_begin(FlowHipCtx* h, const char* name) {
+  if (!h->timing) return -1;
+  if (h->npending == FLOWHIP_MAX_PENDING) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    flowhip_timer_collect(h);
+  }
+  const int t = timer_index(h, name);
+  if (t < 0) return -1;
+  const int slot = h->npending++;
+  h->pending_timer[slot] = t;
+  HIPCHECK(hipEventRecord(h->ev_start[slot], h->stream));
+  return slot;
+}
+
+void flowhip_timer_end(FlowHipCtx* h, int slot) {
+  if (slot < 0) return;
+  HIPCHECK(hipEventRecord(h->ev_stop[slot], h->stream));
+}
+
+void flowhip_timer_collect(FlowHipCtx* h) {
+  for (int k = 0; k < h->npending; ++k) {
+    float ms = 0.0f;
+    HIPCHECK(hipEventSynchronize(h->ev_stop[k]));
+    HIPCHECK(hipEventElapsedTime(&ms, h->ev_start[k], h->ev_stop[k]));
+    FlowHipTimer* t = &h->timers[h->pending_timer[k]];
+    t->total_ms += ms;
+    t->launches += 1;
+  }
+  h->npending = 0;
+}
+
+int flowhip_set_timing(Mesh* mesh, int on) {
+  FlowHipCtx* h = flowhip_ctx(mesh);
+  if (on && !h->events_created) {
+    for (int k = 0; k < FLOWHIP_MAX_PENDING; ++k) {
+      HIPCHECK(hipEventCreate(&h->ev_start[k]));
+      HIPCHECK(hipEventCreate(&h->ev_stop[k]));
+    }
+    h->events_created = 1;
+  }
+  if (!on && h->npending) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    flowhip_timer_collect(h);
+  }
+  h->timing = on ? 1 : 0;
+  return 0;
+}
+
+int flowhip_timing_count(Mesh* mesh) { return flowhip_ctx(mesh)->ntimers; }
+
+int flowhip_timing_entry(Mesh* mesh, int k, char* name, int name_len,
+                         double* total_ms, long* launches) {
+  FlowHipCtx* h = flowhip_ctx(mesh);
+  if (h->npending) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    flowhip_timer_collect(h);
+  }
+  if (k < 0 || k >= h->ntimers) return -1;
+  if (name && name_len > 0) {
+    strncpy(name, h->timers[k].name, (size_t)name_len - 1);
+    name[name_len - 1] = '\0';
+  }
+  if (total_ms) *total_ms = h->timers[k].total_ms;
+  if (launches) *launches = h->timers[k].launches;
+  return 0;
+}
+
+int flowhip_timing_reset(Mesh* mesh) {
+  FlowHipCtx* h = flowhip_ctx(mesh);
+  if (h->npending) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    flowhip_timer_collect(h);
+  }
+  h->ntimers = 0;
+  return 0;
+}

@@ -1,0 +1,79 @@
+/*
+ * flowhip_ctx.h -- private per-Mesh state of the hip kernel set: its stream,
+ * the device-resident timestep pair, reduction scratch, pinned staging for
+ * the one device-to-host transfer per step, captured step graphs and the
+ * per-kernel event timers.
+ */
+#ifndef FLOWHIP_CTX_H
+#define FLOWHIP_CTX_H
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../../include/flow_arch.h"
+#include "../../../include/flow_hip.h"
+#include "../../../include/flow_hip_kernels.h"
+
+#define FLOWHIP_MAX_TIMERS 48
+#define FLOWHIP_MAX_PENDING 256
+
+#define HIPCHECK(call)                                                     \
+  do {                                                                     \
+    hipError_t err_ = (call);                                              \
+    if (err_ != hipSuccess)                                                \
+      TERMINATE("hip kernel set: %s failed at %s:%d: %s", #call, __FILE__, \
+                __LINE__, hipGetErrorString(err_));                        \
+  } while (0)
+
+typedef struct FlowHipTimer {
+  const char* name;
+  double total_ms;
+  long launches;
+} FlowHipTimer;
+
+typedef struct FlowHipCtx {
+  int device;
+  hipStream_t stream;
+  FkReduce red;         /* device scratch */
+  double* h_stage;      /* pinned: [0..1] dt pair, [2..5] sums */
+  int* h_status;        /* pinned */
+  int schedule;
+  int use_graphs;
+  hipGraphExec_t graph[3]; /* even, odd, first step */
+  const void* graph_key[3][16];
+  /* conservation sums fused into the last advection kernel, valid while
+   * the arrays they were taken from are unchanged */
+  int sums_valid;
+  const double* sums_rho;
+  const double* sums_e;
+  /* timing */
+  int timing;
+  int ntimers;
+  FlowHipTimer timers[FLOWHIP_MAX_TIMERS];
+  int npending;
+  int pending_timer[FLOWHIP_MAX_PENDING];
+  hipEvent_t ev_start[FLOWHIP_MAX_PENDING], ev_stop[FLOWHIP_MAX_PENDING];
+  int events_created;
+} FlowHipCtx;
+
+FlowHipCtx* flowhip_ctx(Mesh* mesh);
+void flowhip_ctx_destroy(Mesh* mesh);
+FkMesh flowhip_fkmesh(const Mesh* mesh);
+
+/* Timing brackets around one launch; no-ops unless timing is on. */
+int flowhip_timer_begin(FlowHipCtx* h, const char* name);
+void flowhip_timer_end(FlowHipCtx* h, int slot);
+/* Folds completed event pairs into the timers; call after a sync. */
+void flowhip_timer_collect(FlowHipCtx* h);
+
+/* Launch with the kernel set's error check and optional timing. */
+#define FK_LAUNCH(h, name, call)                  \
+  do {                                            \
+    const int slot_ = flowhip_timer_begin(h, name); \
+    HIPCHECK(call);                               \
+    flowhip_timer_end(h, slot_);                  \
+  } while (0)
+
+/* Invalidate cached conservation sums (any write to device state). */
+void flowhip_invalidate_sums(Mesh* mesh);
+
+#endif

@@ -1,0 +1,213 @@
+/*
+ * hydro_math.h -- the pointwise arithmetic of flow's hydro timestep, shared by
+ * every kernel of the hip set (stagewise and fused, FP64 and FP32 state).
+ *
+ * Each function evaluates one reference expression with its association
+ * order and its divisions kept as written (omp3/flow.c lines cited). The
+ * kernels are compiled with -ffp-contract=off and without fast-math, so in
+ * FP64 every result rounds exactly as the omp3 loop it follows.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace flowhip {
+
+constexpr double kGam = 1.4;          /* flow_interface.h:7 */
+constexpr double kCQ = 3.0;           /* flow_interface.h:8 */
+constexpr double kCT = 0.3;           /* flow_data.h:6 */
+constexpr double kCM = 1.5 / kCT;     /* flow_interface.h:9 */
+
+__device__ __forceinline__ double vabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
+
+/* arch min(a, b) = a < b ? a : b */
+template <typename T>
+__device__ __forceinline__ T lesser(T a, T b) {
+  return (a < b) ? a : b;
+}
+
+/* arch minmod(a, b) = (a*b > 0) ? (|a| < |b| ? a : b) : 0 */
+template <typename T>
+__device__ __forceinline__ T minmod(T a, T b) {
+  return (a * b > T(0.0)) ? ((vabs(a) < vabs(b)) ? a : b) : T(0.0);
+}
+
+/* omp3/flow.c:64-65 */
+template <typename T>
+__device__ __forceinline__ T eos_pressure(T rho, T e) {
+  return T(kGam - 1.0) * rho * e;
+}
+
+/* Edge-centred density on an x face between cells j-1 (w) and j (c):
+ * omp3/flow.c:134-137. */
+template <typename T>
+__device__ __forceinline__ T rho_edge_x(T rho_c, T rho_w, T cdx_c, T cdx_w,
+                                        T cdy, T edx) {
+  return (rho_c * cdx_c * cdy + rho_w * cdx_w * cdy) / (T(2.0) * edx * cdy);
+}
+
+/* Edge-centred density on a y face between cells i-1 (s) and i (c):
+ * omp3/flow.c:138-141. */
+template <typename T>
+__device__ __forceinline__ T rho_edge_y(T rho_c, T rho_s, T cdx, T cdy_c,
+                                        T cdy_s, T edy) {
+  return (rho_c * cdx * cdy_c + rho_s * cdx * cdy_s) / (T(2.0) * cdx * edy);
+}
+
+/* Velocity from momentum with the zero-density guard, omp3/flow.c:144-150. */
+template <typename T>
+__device__ __forceinline__ T velocity(T mom, T rho_edge) {
+  return (rho_edge == T(0.0)) ? T(0.0) : mom / rho_edge;
+}
+
+/* mom -= dt * (a - b) / d: the pressure (omp3/flow.c:126-131) and viscous
+ * (:216-219) accelerations. */
+template <typename T>
+__device__ __forceinline__ T accelerate(T mom, T dt, T a, T b, T d) {
+  return mom - dt * (a - b) / d;
+}
+
+/* mom -= dt_h * (a - b) / (d1 * d2): the momentum advection updates
+ * (omp3/flow.c:581-584, 614-617, 634-637, 664-667). */
+template <typename T>
+__device__ __forceinline__ T advect_update(T mom, T dt_h, T a, T b, T d1,
+                                           T d2) {
+  return mom - dt_h * (a - b) / (d1 * d2);
+}
+
+/* Artificial viscous stress from the four face velocities w0..w3 around a
+ * cell (faces c-1, c, c+1, c+2 along the stress direction):
+ * omp3/flow.c:177-200. */
+template <typename T>
+__device__ __forceinline__ T viscous_stress(T rho, T w0, T w1, T w2, T w3) {
+  const T d_m = lesser(T(0.0), w1 - w0);
+  const T d_0 = lesser(T(0.0), w2 - w1);
+  const T d_p = lesser(T(0.0), w3 - w2);
+  const T q_ii = T(0.5) * (vabs(d_p - d_0) + vabs(d_0 - d_m));
+  return T(-kCQ) * rho * d_0 * q_ii;
+}
+
+/* Shock heating and work on one cell, omp3/flow.c:262-283. u0/u1 are the
+ * west/east face velocities, v0/v1 the south/north ones. */
+template <typename T>
+__device__ __forceinline__ T shock_heating(T e, T P, T rho, T qxx, T qyy,
+                                           T u0, T u1, T v0, T v1, T cdx,
+                                           T cdy, T dt_h) {
+  const T dvx = (u1 - u0) / cdx;
+  const T dvy = (v1 - v0) / cdy;
+  const T div_dt = (dvx + dvy) * dt_h;
+  const T e_q = e - dt_h * (qxx * dvx + qyy * dvy) / rho;
+  const T rho_c = rho / (T(1.0) + div_dt);
+  const T e_c = e_q - (P * div_dt) / rho;
+  const T work = T(0.5) * div_dt * (P + T(kGam - 1.0) * e_c * rho_c) / rho;
+  return (rho == T(0.0)) ? T(0.0) : e_q - work;
+}
+
+/* CFL candidate of one cell, omp3/flow.c:87-94. Always FP64. */
+__device__ __forceinline__ double cell_timestep(double e, double qxx,
+                                                double qyy, double rho,
+                                                double cdx, double cdy) {
+  const double c_s = __builtin_sqrt(kGam * (kGam - 1.0) * e);
+  const double dtx = cdx / __builtin_sqrt(c_s * c_s + 2.0 * qxx / rho);
+  const double dty = cdy / __builtin_sqrt(c_s * c_s + 2.0 * qyy / rho);
+  return lesser(dtx, dty);
+}
+
+/* Monotonised-central upwind interpolation from four consecutive samples
+ * w0..w3, the point lying between w1 and w2. This one pattern is the energy
+ * face value of the mass/energy fluxes (omp3/flow.c:378-399, 494-515) and
+ * the four momentum fluxes (:793-819, :840-870, :894-921, :942-966). */
+template <typename T>
+__device__ __forceinline__ T mc_upwind(T w0, T w1, T w2, T w3, T inv, T vel,
+                                       bool upwind_low, T cd_lo, T cd_hi,
+                                       T dt_h) {
+  if (upwind_low) {
+    const T a0 = T(0.5) * inv * (w2 - w0);
+    const T b0 = T(2.0) * inv * (w1 - w0);
+    const T c0 = T(2.0) * inv * (w2 - w1);
+    return w1 + T(0.5) * minmod(minmod(a0, b0), c0) * (cd_lo - vel * dt_h);
+  }
+  const T a1 = T(0.5) * inv * (w3 - w1);
+  const T b1 = T(2.0) * inv * (w2 - w1);
+  const T c1 = T(2.0) * inv * (w3 - w2);
+  return w2 - T(0.5) * minmod(minmod(a1, b1), c1) * (cd_hi + vel * dt_h);
+}
+
+template <typename T>
+struct FaceFlux {
+  T mass;
+  T energy;
+};
+
+/* Mass and energy flux through one face of a directional sweep,
+ * omp3/flow.c:344-403 (x) and :460-519 (y).
+ *   vm, v0, vp   face velocities at f-1, f, f+1
+ *   rmm..rpp     densities of cells f-2, f-1, f, f+1
+ *   emm..epp     specific energies of the same cells
+ *   ed           face spacing at f; cd_lo, cd_hi cell spacings of f-1, f
+ *   et           transverse edge length (edgedy[i] in x, edgedx[j] in y)
+ * The y sweep selects the Van Leer smoothness by the face velocity v0
+ * instead of the time-centred one (omp3/flow.c:477 vs :361). */
+template <typename T, bool kYSweep>
+__device__ __forceinline__ FaceFlux<T> mass_energy_flux(
+    T vm, T v0, T vp, T rmm, T rm, T rp, T rpp, T emm, T em, T ep, T epp,
+    T ed, T cd_lo, T cd_hi, T et, T dt, T dt_h) {
+  const T inv = T(1.0) / ed;
+  const T s_c = T(0.5) * inv * (vp - vm);
+  const T s_r = T(2.0) * inv * (v0 - vm);
+  const T s_l = T(2.0) * inv * (vp - v0);
+  const T v_tc = v0 - T(0.5) * v0 * dt * minmod(s_c, minmod(s_r, s_l));
+
+  T limiter = T(0.0);
+  const T drho = (rp - rm);
+  if (drho != T(0.0)) {
+    const T sel = kYSweep ? v0 : v_tc;
+    const T r = (sel >= T(0.0)) ? (rm - rmm) / drho : (rpp - rp) / drho;
+    limiter = (r + vabs(r)) / (T(1.0) + vabs(r));
+  }
+  const T rho_up = (v_tc >= T(0.0)) ? rm : rp;
+  FaceFlux<T> out;
+  out.mass = (v_tc * rho_up + T(0.5) * vabs(v_tc) *
+                                  (T(1.0) - vabs((v_tc * dt_h) / cd_hi)) *
+                                  limiter * drho);
+  const T e_face = mc_upwind(emm, em, ep, epp, inv, v_tc, v_tc > T(0.0),
+                             cd_lo, cd_hi, dt_h);
+  out.energy = et * e_face * out.mass;
+  return out;
+}
+
+/* Conservative update of one cell by a directional sweep,
+ * omp3/flow.c:416-432 (x) and :532-547 (y). f_lo/f_hi are the fluxes
+ * through the low/high faces and et_lo/et_hi their edge lengths. */
+template <typename T>
+struct CellState {
+  T rho;
+  T e;
+};
+template <typename T>
+__device__ __forceinline__ CellState<T> sweep_update(
+    T rho, T rho_old, T e, T f_lo, T f_hi, T ef_lo, T ef_hi, T et_lo, T et_hi,
+    T cdx, T cdy, T dt_h, bool first) {
+  CellState<T> out;
+  out.rho = rho - dt_h * (et_hi * f_hi - et_lo * f_lo) / (cdx * cdy);
+  const T rho_e = (rho_old * e - (dt_h * (ef_hi - ef_lo)) / (cdx * cdy));
+  out.e = first ? ((rho_old == T(0.0)) ? T(0.0) : rho_e / rho_old)
+                : ((out.rho == T(0.0)) ? T(0.0) : rho_e / out.rho);
+  return out;
+}
+
+/* Momentum flux = advected mass flux x interpolated velocity:
+ * F = len * 0.5 * (Fa + Fb); flux = F * w_interp (omp3/flow.c:809-820 etc.). */
+template <typename T>
+__device__ __forceinline__ T momentum_flux(T len, T fa, T fb, T w0, T w1,
+                                           T w2, T w3, T ed, T vel, T cd_lo,
+                                           T cd_hi, T dt_h) {
+  const T inv = T(1.0) / ed;
+  const T Fc = len * T(0.5) * (fa + fb);
+  const T w = mc_upwind(w0, w1, w2, w3, inv, vel, vel > T(0.0), cd_lo, cd_hi,
+                        dt_h);
+  return Fc * w;
+}
+
+}  // namespace flowhip

@@ -1,0 +1,249 @@
+"""ctypes binding of libflowhip.so (include/flow_interface.h, flow_arch.h,
+flow_hip.h). The library is the only compute path: if it is missing this
+module raises instead of falling back to anything else."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib", "libflowhip.so")
+
+STAGEWISE, FUSED = 0, 1
+NO_INVERT, INVERT_X, INVERT_Y = 0, 1, 2
+RECV, SEND = 0, 1
+
+# solve_hydro_2d's array arguments in order (flow_interface.h:16-20).
+ARRAY_NAMES = ("P", "rho", "rho_old", "e", "u", "v", "mom_x", "mom_y", "Qxx",
+               "Qyy", "F_x", "F_y", "uF_x", "uF_y", "vF_x", "vF_y")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MeshStruct(ctypes.Structure):
+    """Mirror of `Mesh` in include/flow_arch.h."""
+    _fields_ = [("local_nx", ctypes.c_int), ("local_ny", ctypes.c_int),
+                ("global_nx", ctypes.c_int), ("global_ny", ctypes.c_int),
+                ("niters", ctypes.c_int), ("pad", ctypes.c_int),
+                ("rank", ctypes.c_int), ("nranks", ctypes.c_int),
+                ("x_off", ctypes.c_int), ("y_off", ctypes.c_int),
+                ("neighbours", ctypes.c_int * 4),
+                ("width", ctypes.c_double), ("height", ctypes.c_double),
+                ("max_dt", ctypes.c_double), ("sim_end", ctypes.c_double),
+                ("dt", ctypes.c_double), ("dt_h", ctypes.c_double),
+                ("edgex", _dp), ("edgey", _dp), ("edgedx", _dp),
+                ("edgedy", _dp), ("celldx", _dp), ("celldy", _dp),
+                ("verbose", ctypes.c_int), ("hip", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def library_path():
+    return _LIB
+
+
+def lib():
+    """Load libflowhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        raise RuntimeError(
+            f"hip kernel set not built: {_LIB} is missing "
+            "(run __graft_entry__.build() or make -C flow_amd/csrc)")
+    L = ctypes.CDLL(_LIB)
+    i, d, vp, sz = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+    mp = ctypes.POINTER(MeshStruct)
+    sig = {
+        "flowhip_mesh_new": (mp, [i, i, d, d, d, d, d]),
+        "flowhip_mesh_delete": (None, [mp]),
+        "flowhip_set_schedule": (i, [mp, i]),
+        "flowhip_get_schedule": (i, [mp]),
+        "flowhip_set_graphs": (i, [mp, i]),
+        "flowhip_set_timing": (i, [mp, i]),
+        "flowhip_timing_count": (i, [mp]),
+        "flowhip_timing_entry": (i, [mp, i, ctypes.c_char_p, i, _dp,
+                                     ctypes.POINTER(ctypes.c_long)]),
+        "flowhip_timing_reset": (i, [mp]),
+        "flowhip_run": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp]),
+        "flowhip_device_sync": (i, []),
+        "flowhip_memset_zero": (i, [vp, sz]),
+        "flowhip_copy": (i, [vp, vp, sz, i]),
+        "flowhip_device_name": (i, [ctypes.c_char_p, i]),
+        "allocate_data": (sz, [ctypes.POINTER(vp), sz]),
+        "deallocate_data": (None, [vp]),
+        "handle_boundary_2d": (None, [i, i, mp, vp, i, i]),
+        "solve_hydro_2d": (None, [mp, i] + [vp] * 17),
+        "set_timestep": (None, [i, i, vp, vp, vp, vp, mp, vp, i, vp, vp]),
+        "print_conservation": (None, [i, i, vp, vp, vp, mp]),
+        "get_int_parameter": (i, [ctypes.c_char_p, ctypes.c_char_p]),
+        "get_double_parameter": (d, [ctypes.c_char_p, ctypes.c_char_p]),
+        "get_key_value_parameter": (i, [ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.c_char_p, _dp,
+                                        ctypes.POINTER(ctypes.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Mesh:
+    """A Mesh from flowhip_mesh_new (main.c:20-40 with arch.params values)."""
+
+    def __init__(self, global_nx, global_ny, width=1.0, height=1.0,
+                 max_dt=1.0e30, sim_end=1.0e30, dt=1.0e-4, verbose=False):
+        self.ptr = lib().flowhip_mesh_new(global_nx, global_ny, width, height,
+                                          max_dt, sim_end, dt)
+        if not self.ptr:
+            raise RuntimeError("flowhip_mesh_new failed")
+        self.ptr.contents.verbose = 1 if verbose else 0
+
+    def close(self):
+        if self.ptr:
+            lib().flowhip_mesh_delete(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def s(self):
+        return self.ptr.contents
+
+    nx = property(lambda self: self.s.local_nx)
+    ny = property(lambda self: self.s.local_ny)
+    pad = property(lambda self: self.s.pad)
+    dt = property(lambda self: self.s.dt)
+    dt_h = property(lambda self: self.s.dt_h)
+
+    def set_dt(self, dt, dt_h):
+        self.s.dt = dt
+        self.s.dt_h = dt_h
+
+    def set_schedule(self, schedule):
+        if lib().flowhip_set_schedule(self.ptr, schedule) != 0:
+            raise ValueError(f"bad schedule {schedule}")
+
+    def set_graphs(self, on):
+        lib().flowhip_set_graphs(self.ptr, 1 if on else 0)
+
+    def set_timing(self, on):
+        lib().flowhip_set_timing(self.ptr, 1 if on else 0)
+
+    def timing(self):
+        out = {}
+        name = ctypes.create_string_buffer(128)
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        for k in range(lib().flowhip_timing_count(self.ptr)):
+            lib().flowhip_timing_entry(self.ptr, k, name, 128, ctypes.byref(ms),
+                                       ctypes.byref(n))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
+
+    def reset_timing(self):
+        lib().flowhip_timing_reset(self.ptr)
+
+
+class DeviceState:
+    """The 16 device arrays solve_hydro_2d takes, each (L+1)^2 doubles
+    allocated zeroed by allocate_data (flow_data.c:8-15)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.n = (mesh.nx + 1) * (mesh.ny + 1)
+        self.ptrs = {}
+        for k in ARRAY_NAMES:
+            p = ctypes.c_void_p()
+            lib().allocate_data(ctypes.byref(p), self.n)
+            self.ptrs[k] = p.value
+
+    def close(self):
+        for k, p in list(self.ptrs.items()):
+            lib().deallocate_data(ctypes.c_void_p(p))
+        self.ptrs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, name, host):
+        host = np.ascontiguousarray(host, dtype=np.float64).ravel()
+        buf = np.zeros(self.n, dtype=np.float64)
+        buf[: host.size] = host
+        lib().flowhip_copy(ctypes.c_void_p(self.ptrs[name]), _ptr(buf), self.n, SEND)
+
+    def download(self, name):
+        buf = np.empty(self.n, dtype=np.float64)
+        lib().flowhip_copy(ctypes.c_void_p(self.ptrs[name]), _ptr(buf), self.n, RECV)
+        return buf
+
+    def download_all(self):
+        return {k: self.download(k) for k in ARRAY_NAMES}
+
+    def zero(self, name):
+        lib().flowhip_memset_zero(ctypes.c_void_p(self.ptrs[name]), self.n)
+
+    def boundary(self, name, nxp, nyp, invert):
+        lib().handle_boundary_2d(nxp, nyp, self.mesh.ptr,
+                                 ctypes.c_void_p(self.ptrs[name]), invert, 1)
+
+    def reflect_initial(self):
+        """main.c:56-59: reflect density and energy before the first step."""
+        m = self.mesh
+        self.boundary("rho", m.nx, m.ny, NO_INVERT)
+        self.boundary("e", m.nx, m.ny, NO_INVERT)
+
+    def array_ptrs(self):
+        return (ctypes.c_void_p * 16)(*[self.ptrs[k] for k in ARRAY_NAMES])
+
+    def solve(self, tt):
+        lib().solve_hydro_2d(self.mesh.ptr, tt,
+                             *[ctypes.c_void_p(self.ptrs[k]) for k in ARRAY_NAMES],
+                             None)
+
+    def set_timestep(self, first_step):
+        s = self.mesh.s
+        lib().set_timestep(s.local_nx, s.local_ny,
+                           ctypes.c_void_p(self.ptrs["Qxx"]),
+                           ctypes.c_void_p(self.ptrs["Qyy"]),
+                           ctypes.c_void_p(self.ptrs["rho"]),
+                           ctypes.c_void_p(self.ptrs["e"]), self.mesh.ptr, None,
+                           first_step, ctypes.cast(s.celldx, ctypes.c_void_p),
+                           ctypes.cast(s.celldy, ctypes.c_void_p))
+
+    def run(self, niters, tt0=0, pre_timestep=True):
+        """main.c:64-115 through flowhip_run. Returns (steps, dt_pairs,
+        mass, energy, wall_seconds)."""
+        dt = np.zeros(2 * (niters + 1))
+        mass = np.zeros(max(niters, 1))
+        energy = np.zeros(max(niters, 1))
+        wall = ctypes.c_double()
+        steps = lib().flowhip_run(self.mesh.ptr, self.array_ptrs(), niters, tt0,
+                                  1 if pre_timestep else 0, _ptr(dt),
+                                  _ptr(mass), _ptr(energy), ctypes.byref(wall))
+        return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
+                energy[:steps], wall.value)
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    lib().flowhip_device_name(buf, 256)
+    return buf.value.decode()
+
+
+def sync():
+    lib().flowhip_device_sync()

@@ -1,0 +1,117 @@
+/*
+ * flow_hip_kernels.h -- thin C ABI between the hip kernel set's C host code
+ * (flow_amd/csrc/hip/flow.c) and its HIP kernels (flow_amd/csrc/kernels/).
+ *
+ * Plain device pointers, sizes and a hipStream_t; every launcher returns 0
+ * or a hipError_t. Each launcher names the omp3 loop it replaces. Arrays use
+ * the reference layouts (cell i*nx+j; x-face i*(nx+1)+j; y-face i*nx+j with
+ * ny+1 rows; corner i*(nx+1)+j with ny+1 rows), nx/ny local sizes with pad.
+ *
+ * Timestep scalars stay on the device: `dtp` points at {dt, dt_h}. Kernels
+ * launched before fk_timestep read the previous pair, kernels after it the
+ * new one, by stream order.
+ */
+#ifndef FLOW_HIP_KERNELS_H
+#define FLOW_HIP_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct FkMesh {
+  int nx, ny, pad;
+  const double *edgedx, *edgedy, *celldx, *celldy; /* device, L+1 each */
+} FkMesh;
+
+/* Device scratch for the single-pass grid reductions. */
+typedef struct FkReduce {
+  double* partials;      /* fk_reduce_blocks() doubles */
+  unsigned int* ticket;  /* one counter, zero between launches */
+  double* dtp;           /* {dt, dt_h} */
+  double* sums;          /* {mass, energy, unclamped min dt, spare} */
+  int* status;           /* bit 0: non-finite dt candidate seen */
+  int max_blocks;
+} FkReduce;
+
+/* Number of doubles FkReduce.partials must hold. */
+int fk_reduce_blocks(void);
+
+/* arch handle_boundary_2d, on-device (omp3/flow.c:156 etc.). */
+hipError_t fk_boundary(int nxp, int nyp, int pad, double* arr, int invert,
+                       hipStream_t s);
+hipError_t fk_copy(long n, double* dst, const double* src, hipStream_t s);
+
+/* ---- stagewise schedule: one kernel per omp3 loop ---- */
+/* omp3/flow.c:59-67 */
+hipError_t fk_eos(FkMesh m, double* P, const double* rho, const double* e,
+                  hipStream_t s);
+/* omp3/flow.c:121-152 (dt = dtp[0]) */
+hipError_t fk_pressure_accel(FkMesh m, const double* dtp, double* mom_x,
+                             double* mom_y, double* u, double* v,
+                             const double* P, const double* rho, hipStream_t s);
+/* omp3/flow.c:173-202 */
+hipError_t fk_av_stress(FkMesh m, double* Qxx, double* Qyy, const double* u,
+                        const double* v, const double* rho, hipStream_t s);
+/* omp3/flow.c:212-240 (dt = dtp[0]) */
+hipError_t fk_av_accel(FkMesh m, const double* dtp, double* mom_x,
+                       double* mom_y, double* u, double* v, const double* Qxx,
+                       const double* Qyy, const double* rho, hipStream_t s);
+/* omp3/flow.c:258-285 (dt_h = dtp[1]) */
+hipError_t fk_shock_heating(FkMesh m, const double* dtp, double* e,
+                            const double* P, const double* u, const double* v,
+                            const double* rho, const double* Qxx,
+                            const double* Qyy, hipStream_t s);
+/* omp3/flow.c:77-105: grid minimum of the CFL candidates, seeded with
+ * max_dt, then the (dt, dt_h) update written to r->dtp. One launch. */
+hipError_t fk_timestep(FkMesh m, const double* Qxx, const double* Qyy,
+                       const double* rho, const double* e, FkReduce r,
+                       double max_dt, int first_step, hipStream_t s);
+/* omp3/flow.c:338-404 / 454-521 (dt = dtp[0], dt_h = dtp[1]) */
+hipError_t fk_x_flux(FkMesh m, const double* dtp, const double* rho,
+                     const double* e, const double* u, double* F_x,
+                     double* eF_x, hipStream_t s);
+hipError_t fk_y_flux(FkMesh m, const double* dtp, const double* rho,
+                     const double* e, const double* v, double* F_y,
+                     double* eF_y, hipStream_t s);
+/* omp3/flow.c:412-434 / 528-549 */
+hipError_t fk_x_update(FkMesh m, const double* dtp, int first, double* rho,
+                       const double* rho_old, double* e, const double* F_x,
+                       const double* eF_x, hipStream_t s);
+hipError_t fk_y_update(FkMesh m, const double* dtp, int first, double* rho,
+                       const double* rho_old, double* e, const double* F_y,
+                       const double* eF_y, hipStream_t s);
+/* omp3/flow.c:778-823, 826-874, 877-925, 928-970 (dt_h = dtp[1]) */
+hipError_t fk_mom_flux_xx(FkMesh m, const double* dtp, const double* u,
+                          double* uF_x, const double* F_x, hipStream_t s);
+hipError_t fk_mom_flux_xy(FkMesh m, const double* dtp, const double* u,
+                          const double* v, double* uF_y, const double* F_y,
+                          hipStream_t s);
+hipError_t fk_mom_flux_yx(FkMesh m, const double* dtp, const double* u,
+                          const double* v, double* vF_x, const double* F_x,
+                          hipStream_t s);
+hipError_t fk_mom_flux_yy(FkMesh m, const double* dtp, const double* v,
+                          double* vF_y, const double* F_y, hipStream_t s);
+/* Momentum sub-updates of omp3/flow.c:577-594 (flux_dir 0: uF_x),
+ * :610-618 and :683-700 (flux_dir 1: uF_y); recompute selects the velocity
+ * recompute that follows the first sub-update of each parity. */
+hipError_t fk_mom_x_update(FkMesh m, const double* dtp, int flux_dir,
+                           int recompute, double* mom_x, double* u,
+                           const double* flux, const double* rho,
+                           hipStream_t s);
+/* omp3/flow.c:630-647, 660-669, 731-748, 763-771 (flux_dir 0: vF_x,
+ * flux_dir 1: vF_y) */
+hipError_t fk_mom_y_update(FkMesh m, const double* dtp, int flux_dir,
+                           int recompute, double* mom_y, double* v,
+                           const double* flux, const double* rho,
+                           hipStream_t s);
+/* omp3/flow.c:979-986: compensated FP64 interior sums of rho and rho*e
+ * into r->sums[0..1]. One launch. */
+hipError_t fk_conservation(FkMesh m, const double* rho, const double* e,
+                           FkReduce r, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""Parity of the hip kernel set with the oracle on identical inputs.
+
+The FP64 bar: rho, e, u and v within relative L-inf
+1e-11 after N steps, the dt sequence within 1e-12 relative, the mass and
+energy totals to round-off. The kernels keep omp3's expression trees with
+contraction off, so the tests also count bitwise differences, which are
+expected to be zero.
+"""
+import numpy as np
+import pytest
+
+import flow_amd
+from flow_amd import problems
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-11
+DT_TOL = 1e-12
+SUM_TOL = 1e-13
+
+CASES = [
+    ("sod_x", 32, 3), ("sod_y", 32, 3), ("square_blast", 40, 4),
+    ("multi_blast", 64, 5), ("multi_blast", 100, 2), ("flow_params", 50, 6),
+]
+
+
+def _check(got, ref, L, bitwise):
+    cmp = harness.compare_state(got["arrays"], ref["arrays"], L)
+    for k, (rel, ndiff) in cmp.items():
+        print(f"  {k:6s} rel_linf={rel:.3e} bitwise_diffs={ndiff}", flush=True)
+        assert rel <= FIELD_TOL, (k, rel)
+        if bitwise:
+            assert ndiff == 0, (k, ndiff)
+    assert got["steps"] == ref["steps"]
+    dt_rel = np.max(np.abs(got["dt"] - ref["dt"]) / np.abs(ref["dt"]))
+    assert dt_rel <= DT_TOL, dt_rel
+    if bitwise:
+        np.testing.assert_array_equal(got["dt"], ref["dt"])
+    np.testing.assert_allclose(got["mass"], ref["mass"], rtol=SUM_TOL)
+    np.testing.assert_allclose(got["energy"], ref["energy"], rtol=SUM_TOL)
+
+
+@pytest.mark.parametrize("name,n,steps", CASES)
+def test_stagewise_matches_oracle_bitwise(name, n, steps):
+    """The stagewise schedule writes every array omp3 writes: all 16 arrays
+    must equal the oracle's bit for bit, ghosts and corners included."""
+    print(f"case stagewise {name} n={n} steps={steps}", flush=True)
+    fn = problems.PROBLEMS[name]
+    ref = harness.oracle_run(fn, n, steps)
+    got = harness.gpu_run(fn, n, steps, flow_amd.STAGEWISE)
+    _check(got, ref, ref["L"], bitwise=True)
+    for k in flow_amd.ARRAY_NAMES:
+        diff = int(np.count_nonzero(got["arrays"][k] != ref["arrays"][k]))
+        assert diff == 0, (k, diff)
+
+
+def test_stagewise_kat_flow_params():
+    """flow.tests:1 on the GPU: flow.params, 4096^2, 100 iterations."""
+    import json, os
+    here = os.path.dirname(os.path.abspath(__file__))
+    kat = json.load(open(os.path.join(here, "golden", "flow_tests_kat.json")))
+    n = kat["nx"]
+    print(f"case stagewise KAT n={n}", flush=True)
+    got = harness.gpu_run(problems.flow_params_problem, n, kat["iterations"],
+                          flow_amd.STAGEWISE, dt=kat["dt"])
+    L = got["L"]
+    v = harness.views(got["arrays"], L)
+    dsum = float(v["rho"][2:L - 2, 2:L - 2].sum())
+    esum = float(v["e"][2:L - 2, 2:L - 2].sum())
+    print(f"  density {dsum:.12e} energy {esum:.12e}", flush=True)
+    assert "%.12e" % dsum == "%.12e" % kat["density"]
+    assert "%.12e" % esum == "%.12e" % kat["energy"]
