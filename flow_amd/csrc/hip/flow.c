@@ -26,17 +26,17 @@ static void push_dt(FlowHipCtx* h, const Mesh* mesh) {
 static void pull_dt(FlowHipCtx* h, Mesh* mesh) {
   HIPCHECK(hipMemcpyAsync(h->h_stage, h->red.dtp, 2 * sizeof(double),
                           hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(h->h_status, h->red.status, sizeof(int),
+  HIPCHECK(hipMemcpyAsync(h->h_status, h->red.status, 2 * sizeof(int),
                           hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   flowhip_timer_collect(h);
   mesh->dt = h->h_stage[0];
   mesh->dt_h = h->h_stage[1];
-  if (h->h_status[0] || !isfinite(mesh->dt) || !(mesh->dt > 0.0))
+  if (!isfinite(mesh->dt) || !(mesh->dt > 0.0) || !isfinite(mesh->dt_h))
     TERMINATE("hip kernel set: non-finite or non-positive timestep "
-              "(dt=%.12e dt_h=%.12e): the state holds NaN or negative "
-              "energy",
-              mesh->dt, mesh->dt_h);
+              "(dt=%.12e dt_h=%.12e, %d undefined CFL candidates): the "
+              "state has diverged",
+              mesh->dt, mesh->dt_h, h->h_status[1]);
 }
 
 /* omp3/flow.c:10-52 */

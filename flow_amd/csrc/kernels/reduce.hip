@@ -79,10 +79,14 @@ __global__ __launch_bounds__(kRedThreads) void k_timestep(
     const int c = i * m.nx + j;
     const double cand =
         cell_timestep(e[c], Qxx[c], Qyy[c], rho[c], m.celldx[j], m.celldy[i]);
-    bad |= !(cand == cand);
-    local = lesser(local, cand);
+    /* NaN candidates (cells with negative energy) are left out, as the
+     * reference's min(a, b) leaves them out in effect; they are counted. */
+    if (cand == cand)
+      local = lesser(local, cand);
+    else
+      ++bad;
   }
-  if (bad) atomicOr(r.status, 1);
+  if (bad) atomicAdd(r.status, bad);
   const double bmin = block_min(local);
   if (threadIdx.x == 0) r.partials[blockIdx.x] = bmin;
   if (!last_block_arrives(r.ticket)) return;
@@ -97,6 +101,8 @@ __global__ __launch_bounds__(kRedThreads) void k_timestep(
     r.dtp[0] = dt;
     r.dtp[1] = first_step ? dt : kCT * final_min;
     r.sums[2] = g; /* unclamped grid minimum, kept for tests */
+    /* NaN-candidate count of this launch moves to status[1] */
+    r.status[1] = atomicExch(r.status, 0);
     *r.ticket = 0u;
   }
 }

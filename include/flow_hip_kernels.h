@@ -31,7 +31,8 @@ typedef struct FkReduce {
   unsigned int* ticket;  /* one counter, zero between launches */
   double* dtp;           /* {dt, dt_h} */
   double* sums;          /* {mass, energy, unclamped min dt, spare} */
-  int* status;           /* bit 0: non-finite dt candidate seen */
+  int* status;           /* [0] NaN CFL candidates being counted, [1] the
+                            count of the last fk_timestep launch */
   int max_blocks;
 } FkReduce;
 

@@ -149,7 +149,12 @@ double orc_set_timestep(int nx, int ny, const double* Qxx, const double* Qyy,
       const double dty =
           m->celldy[i] / sqrt(c_s * c_s + 2.0 * Qyy[c] / rho[c]);
       const double cell_min = MIN(dtx, dty);
-      local_min = MIN(local_min, cell_min);
+      /* A cell with negative energy gives a NaN candidate (sqrt of a
+       * negative). The reference's min(a, b) = a < b ? a : b drops it unless
+       * it is the last value an OpenMP thread sees, so its effective result
+       * is the minimum over the defined candidates; the oracle takes that
+       * minimum explicitly, independent of the thread count. */
+      if (cell_min == cell_min) local_min = MIN(local_min, cell_min);
     }
   const double final_min = MIN(local_min, C_M * m->dt_h);
   m->dt = 0.5 * (C_T * final_min + m->dt_h);

@@ -19,9 +19,14 @@ FIELD_TOL = 1e-11
 DT_TOL = 1e-12
 SUM_TOL = 1e-13
 
+# (problem, n, steps, .params dt). The last case reproduces what flow.params
+# does at 4096^2: its first step drives e negative in some cells, whose CFL
+# candidates are NaN and must be left out of the dt minimum.
 CASES = [
-    ("sod_x", 32, 3), ("sod_y", 32, 3), ("square_blast", 40, 4),
-    ("multi_blast", 64, 5), ("multi_blast", 100, 2), ("flow_params", 50, 6),
+    ("sod_x", 32, 3, 1e-4), ("sod_y", 32, 3, 1e-4),
+    ("square_blast", 40, 4, 1e-4), ("multi_blast", 64, 5, 1e-5),
+    ("multi_blast", 100, 2, 1e-5), ("flow_params", 50, 6, 1e-4),
+    ("flow_params", 64, 4, 0.064),
 ]
 
 
@@ -41,14 +46,14 @@ def _check(got, ref, L, bitwise):
     np.testing.assert_allclose(got["energy"], ref["energy"], rtol=SUM_TOL)
 
 
-@pytest.mark.parametrize("name,n,steps", CASES)
-def test_stagewise_matches_oracle_bitwise(name, n, steps):
+@pytest.mark.parametrize("name,n,steps,dt", CASES)
+def test_stagewise_matches_oracle_bitwise(name, n, steps, dt):
     """The stagewise schedule writes every array omp3 writes: all 16 arrays
     must equal the oracle's bit for bit, ghosts and corners included."""
-    print(f"case stagewise {name} n={n} steps={steps}", flush=True)
+    print(f"case stagewise {name} n={n} steps={steps} dt={dt}", flush=True)
     fn = problems.PROBLEMS[name]
-    ref = harness.oracle_run(fn, n, steps)
-    got = harness.gpu_run(fn, n, steps, flow_amd.STAGEWISE)
+    ref = harness.oracle_run(fn, n, steps, dt=dt)
+    got = harness.gpu_run(fn, n, steps, flow_amd.STAGEWISE, dt=dt)
     _check(got, ref, ref["L"], bitwise=True)
     for k in flow_amd.ARRAY_NAMES:
         diff = int(np.count_nonzero(got["arrays"][k] != ref["arrays"][k]))
