@@ -31,7 +31,7 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
                          hipHostMallocDefault));
   memset(h->h_stage, 0, sizeof(double) * 8);
   memset(h->h_status, 0, sizeof(int) * 4);
-  h->schedule = FLOWHIP_STAGEWISE;
+  h->schedule = FLOWHIP_FUSED;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   return h;

@@ -47,6 +47,11 @@ void solve_hydro_2d(Mesh* mesh, int tt, double* P, double* rho,
                     double* vF_x, double* vF_y, double* reduce_array) {
   (void)reduce_array;
   FlowHipCtx* h = flowhip_ctx(mesh);
+  /* a pad-2 reflection needs two interior cells to mirror along each axis */
+  if (mesh->local_nx - 2 * mesh->pad < 2 || mesh->local_ny - 2 * mesh->pad < 2)
+    TERMINATE("solve_hydro_2d: the mesh needs at least 2x2 interior cells "
+              "(got %dx%d)", mesh->local_nx - 2 * mesh->pad,
+              mesh->local_ny - 2 * mesh->pad);
   if (mesh->rank == MASTER && mesh->verbose)
     printf("Timestep:        %.12e\n", mesh->dt);
 

@@ -107,7 +107,7 @@ int flowhip_device_name(char* name, int name_len) {
 Mesh* flowhip_mesh_new(int global_nx, int global_ny, double width,
                        double height, double max_dt, double sim_end,
                        double dt_param) {
-  if (global_nx < 1 || global_ny < 1) return NULL;
+  if (global_nx < 2 || global_ny < 2) return NULL;
   Mesh* m = (Mesh*)calloc(1, sizeof(Mesh));
   if (!m) return NULL;
   m->global_nx = global_nx;

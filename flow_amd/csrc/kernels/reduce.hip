@@ -1,6 +1,6 @@
 /*
- * reduce.hip -- single-pass grid reductions of the hip kernel set: the CFL
- * timestep minimum (omp3/flow.c:77-105) and the conservation totals
+ * reduce.hip -- single-pass grid reductions of the stagewise schedule: the
+ * CFL timestep minimum (omp3/flow.c:77-105) and the conservation totals
  * (omp3/flow.c:973-995).
  *
  * Each block reduces its share through wave64 shuffles and LDS, writes one
@@ -8,8 +8,7 @@
  * partials, writes the result and resets the ticket to zero, so no memset is
  * needed between launches (or inside a captured graph).
  */
-#include "common.h"
-#include "hydro_math.h"
+#include "reduce_common.h"
 
 namespace flowhip {
 namespace {
@@ -17,54 +16,11 @@ namespace {
 constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = 1024;
 
-/* Returns true in the one block that arrives last. */
-__device__ bool last_block_arrives(unsigned int* ticket) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int prev = atomicAdd(ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  /* acquire: the other blocks' partials become visible to this block */
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ double block_min(double x) {
-  __shared__ double red[kRedThreads / kWave];
-  x = wave_min(x);
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  if (lane == 0) red[w] = x;
-  __syncthreads();
-  if (w == 0) {
-    x = (lane < kRedThreads / kWave) ? red[lane] : red[0];
-    x = wave_min(x);
-  }
-  return x; /* valid in wave 0 */
-}
-
-__device__ CSum block_csum(CSum v) {
-  __shared__ double rs[kRedThreads / kWave], rc[kRedThreads / kWave];
-  v = wave_csum(v);
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  if (lane == 0) {
-    rs[w] = v.s;
-    rc[w] = v.c;
-  }
-  __syncthreads();
-  if (w == 0) {
-    CSum t{0.0, 0.0};
-    if (lane < kRedThreads / kWave) t = CSum{rs[lane], rc[lane]};
-    v = wave_csum(t);
-  }
-  return v; /* valid in wave 0 */
-}
-
-/* CFL minimum over the interior, then the dt pair update of
- * omp3/flow.c:103-105 in the last block. Cells are visited in a grid-stride
- * loop; min is exact, so the visiting order does not change the result. */
+/* CFL minimum over the interior, then the dt pair update in the last block.
+ * min is exact, so the visiting order does not change the result. NaN
+ * candidates (cells with negative energy: sqrt of a negative) are left out,
+ * as the reference's min(a, b) = a < b ? a : b leaves them out in effect,
+ * and counted. */
 __global__ __launch_bounds__(kRedThreads) void k_timestep(
     FkMesh m, const double* __restrict__ Qxx, const double* __restrict__ Qyy,
     const double* __restrict__ rho, const double* __restrict__ e, FkReduce r,
@@ -79,32 +35,12 @@ __global__ __launch_bounds__(kRedThreads) void k_timestep(
     const int c = i * m.nx + j;
     const double cand =
         cell_timestep(e[c], Qxx[c], Qyy[c], rho[c], m.celldx[j], m.celldy[i]);
-    /* NaN candidates (cells with negative energy) are left out, as the
-     * reference's min(a, b) leaves them out in effect; they are counted. */
     if (cand == cand)
       local = lesser(local, cand);
     else
       ++bad;
   }
-  if (bad) atomicAdd(r.status, bad);
-  const double bmin = block_min(local);
-  if (threadIdx.x == 0) r.partials[blockIdx.x] = bmin;
-  if (!last_block_arrives(r.ticket)) return;
-  double g = max_dt;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-    g = lesser(g, r.partials[b]);
-  g = block_min(g);
-  if (threadIdx.x == 0) {
-    const double dt_h = r.dtp[1];
-    const double final_min = lesser(g, kCM * dt_h);
-    const double dt = 0.5 * (kCT * final_min + dt_h);
-    r.dtp[0] = dt;
-    r.dtp[1] = first_step ? dt : kCT * final_min;
-    r.sums[2] = g; /* unclamped grid minimum, kept for tests */
-    /* NaN-candidate count of this launch moves to status[1] */
-    r.status[1] = atomicExch(r.status, 0);
-    *r.ticket = 0u;
-  }
+  finish_timestep<kRedThreads>(local, bad, r, max_dt, first_step);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_conservation(
@@ -120,31 +56,7 @@ __global__ __launch_bounds__(kRedThreads) void k_conservation(
     ms.add(rho[c]);
     es.add(rho[c] * e[c]);
   }
-  ms = block_csum(ms);
-  __syncthreads();
-  es = block_csum(es);
-  if (threadIdx.x == 0) {
-    double* p = r.partials + 4 * blockIdx.x;
-    p[0] = ms.s;
-    p[1] = ms.c;
-    p[2] = es.s;
-    p[3] = es.c;
-  }
-  if (!last_block_arrives(r.ticket)) return;
-  CSum gm{0.0, 0.0}, ge{0.0, 0.0};
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    const double* p = r.partials + 4 * b;
-    gm.merge(CSum{p[0], p[1]});
-    ge.merge(CSum{p[2], p[3]});
-  }
-  gm = block_csum(gm);
-  __syncthreads();
-  ge = block_csum(ge);
-  if (threadIdx.x == 0) {
-    r.sums[0] = gm.s + gm.c;
-    r.sums[1] = ge.s + ge.c;
-    *r.ticket = 0u;
-  }
+  finish_sums<kRedThreads>(ms, es, r);
 }
 
 int red_blocks(long n) {
@@ -160,7 +72,7 @@ using namespace flowhip;
 
 extern "C" {
 
-int fk_reduce_blocks(void) { return 4 * kRedBlocks; }
+int fk_reduce_blocks(void) { return 4 * kMaxReduceBlocks; }
 
 hipError_t fk_timestep(FkMesh m, const double* Qxx, const double* Qyy,
                        const double* rho, const double* e, FkReduce r,

@@ -112,6 +112,45 @@ hipError_t fk_mom_y_update(FkMesh m, const double* dtp, int flux_dir,
 hipError_t fk_conservation(FkMesh m, const double* rho, const double* e,
                            FkReduce r, hipStream_t s);
 
+/* ---- fused schedule: four kernels per step (fused.hip) ----
+ * Each kernel reads only arrays it does not write; the host schedule routes
+ * the intermediate fields through scratch arrays. */
+/* omp3/flow.c:22-38 (EOS, pressure acceleration, artificial viscosity,
+ * shock heating, set_timestep): reads rho, e, mom_x, mom_y and the old
+ * (dt, dt_h) in r.dtp; writes the momenta after the viscous step (mx1, my1,
+ * interior faces), the velocities after it (u0, v0, with ghosts), the
+ * energy after shock heating (e1, with ghosts), rho_old = rho, and the new
+ * (dt, dt_h) into r.dtp. One launch. */
+hipError_t fk_lagrangian(FkMesh m, const double* rho, const double* e,
+                         const double* mom_x, const double* mom_y, double* mx1,
+                         double* my1, double* u0, double* v0, double* e1,
+                         double* rho_old, FkReduce r, double max_dt,
+                         int first_step, hipStream_t s);
+/* omp3/flow.c:293-554: both directional sweeps (x first for even tt),
+ * rho0 = density before advection, e1 = energy after shock heating; writes
+ * rho, e, F_x, F_y (with ghosts) and the conservation totals (omp3/flow.c:
+ * 979-986) into r.sums. */
+hipError_t fk_advect_mass_energy(FkMesh m, int tt, const double* u0,
+                                 const double* v0, const double* rho0,
+                                 const double* e1, double* rho, double* e,
+                                 double* F_x, double* F_y, FkReduce r,
+                                 hipStream_t s);
+/* omp3/flow.c:570-619 (odd tt) / 673-722 (even tt): x momentum mx1 ->
+ * mom_x, and the recomputed x velocity u (with ghosts). */
+hipError_t fk_advect_momentum_x(FkMesh m, int tt, const double* u0,
+                                const double* v0, const double* F_x,
+                                const double* F_y, const double* rho,
+                                const double* mx1, double* mom_x, double* u,
+                                const double* dtp, hipStream_t s);
+/* omp3/flow.c:621-669 (odd tt) / 724-774 (even tt): y momentum my1 ->
+ * mom_y, and the recomputed y velocity v (with ghosts); u1 is the x
+ * velocity fk_advect_momentum_x wrote. */
+hipError_t fk_advect_momentum_y(FkMesh m, int tt, const double* u1,
+                                const double* v0, const double* F_x,
+                                const double* F_y, const double* rho,
+                                const double* my1, double* mom_y, double* v,
+                                const double* dtp, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,3 +76,62 @@ def test_stagewise_kat_flow_params():
     print(f"  density {dsum:.12e} energy {esum:.12e}", flush=True)
     assert "%.12e" % dsum == "%.12e" % kat["density"]
     assert "%.12e" % esum == "%.12e" % kat["energy"]
+
+
+# Fused schedule: the carried state (rho, e, u, v and the momenta on their
+# face domains), the mass fluxes and rho_old equal omp3's bit for bit. The
+# remaining scratch arrays hold the fused kernels' ping-pong fields.
+FUSED_CASES = CASES + [
+    ("multi_blast", 2, 3, 1e-5), ("multi_blast", 3, 4, 1e-5),
+    ("square_blast", 33, 5, 1e-4), ("sod_x", 47, 4, 1e-4),
+    ("multi_blast", 1000, 3, 1e-5),
+]
+
+
+@pytest.mark.parametrize("name,n,steps,dt", FUSED_CASES)
+def test_fused_matches_oracle_bitwise(name, n, steps, dt):
+    print(f"case fused {name} n={n} steps={steps} dt={dt}", flush=True)
+    fn = problems.PROBLEMS[name]
+    ref = harness.oracle_run(fn, n, steps, dt=dt)
+    got = harness.gpu_run(fn, n, steps, flow_amd.FUSED, dt=dt)
+    _check(got, ref, ref["L"], bitwise=True)
+    L = ref["L"]
+    for k in ("F_x", "F_y", "rho_old"):
+        a, b = got["arrays"][k], ref["arrays"][k]
+        if k == "F_x":
+            m = harness.noncorner_mask(L, L + 1).ravel()
+            a, b = a[: L * (L + 1)][m], b[: L * (L + 1)][m]
+        elif k == "F_y":
+            m = harness.noncorner_mask(L + 1, L).ravel()
+            a, b = a[: (L + 1) * L][m], b[: (L + 1) * L][m]
+        diff = int(np.count_nonzero(a != b))
+        print(f"  {k:7s} bitwise_diffs={diff}", flush=True)
+        assert diff == 0, (k, diff)
+
+
+def test_fused_is_deterministic():
+    print("case fused determinism n=300", flush=True)
+    a = harness.gpu_run(problems.multi_blast, 300, 4, flow_amd.FUSED, dt=1e-5)
+    b = harness.gpu_run(problems.multi_blast, 300, 4, flow_amd.FUSED, dt=1e-5)
+    for k in flow_amd.ARRAY_NAMES:
+        assert np.array_equal(a["arrays"][k], b["arrays"][k], equal_nan=True), k
+    np.testing.assert_array_equal(a["mass"], b["mass"])
+    np.testing.assert_array_equal(a["energy"], b["energy"])
+
+
+def test_fused_kat_flow_params():
+    """flow.tests:1 through the fused schedule."""
+    import json, os
+    here = os.path.dirname(os.path.abspath(__file__))
+    kat = json.load(open(os.path.join(here, "golden", "flow_tests_kat.json")))
+    n = kat["nx"]
+    print(f"case fused KAT n={n}", flush=True)
+    got = harness.gpu_run(problems.flow_params_problem, n, kat["iterations"],
+                          flow_amd.FUSED, dt=kat["dt"])
+    L = got["L"]
+    v = harness.views(got["arrays"], L)
+    dsum = float(v["rho"][2:L - 2, 2:L - 2].sum())
+    esum = float(v["e"][2:L - 2, 2:L - 2].sum())
+    print(f"  density {dsum:.12e} energy {esum:.12e}", flush=True)
+    assert "%.12e" % dsum == "%.12e" % kat["density"]
+    assert "%.12e" % esum == "%.12e" % kat["energy"]
