@@ -49,6 +49,8 @@ void flowhip_ctx_destroy(Mesh* mesh) {
       hipEventDestroy(h->ev_stop[k]);
     }
   }
+  hipFree(h->inv_edgedx);
+  hipFree(h->inv_edgedy);
   hipFree(h->red.partials);
   hipFree(h->red.ticket);
   hipFree(h->red.dtp);
@@ -61,8 +63,34 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   mesh->hip = NULL;
 }
 
+/* 1/edgedx and 1/edgedy, recomputed when the geometry arrays change. */
+static void ensure_reciprocals(FlowHipCtx* h, const Mesh* mesh) {
+  const int lx = mesh->local_nx + 1, ly = mesh->local_ny + 1;
+  if (h->inv_src_x == mesh->edgedx && h->inv_src_y == mesh->edgedy &&
+      h->inv_nx == lx && h->inv_ny == ly)
+    return;
+  hipFree(h->inv_edgedx);
+  hipFree(h->inv_edgedy);
+  HIPCHECK(hipMalloc((void**)&h->inv_edgedx, sizeof(double) * lx));
+  HIPCHECK(hipMalloc((void**)&h->inv_edgedy, sizeof(double) * ly));
+  HIPCHECK(fk_reciprocal(lx, h->inv_edgedx, mesh->edgedx, h->stream));
+  HIPCHECK(fk_reciprocal(ly, h->inv_edgedy, mesh->edgedy, h->stream));
+  h->inv_src_x = mesh->edgedx;
+  h->inv_src_y = mesh->edgedy;
+  h->inv_nx = lx;
+  h->inv_ny = ly;
+}
+
 FkMesh flowhip_fkmesh(const Mesh* mesh) {
   FkMesh m;
+  FlowHipCtx* h = (FlowHipCtx*)mesh->hip;
+  if (h) {
+    ensure_reciprocals(h, mesh);
+    m.inv_edgedx = h->inv_edgedx;
+    m.inv_edgedy = h->inv_edgedy;
+  } else {
+    m.inv_edgedx = m.inv_edgedy = NULL;
+  }
   m.nx = mesh->local_nx;
   m.ny = mesh->local_ny;
   m.pad = mesh->pad;

@@ -34,6 +34,9 @@ typedef struct FlowHipCtx {
   int device;
   hipStream_t stream;
   FkReduce red;         /* device scratch */
+  double *inv_edgedx, *inv_edgedy; /* 1/edgedx, 1/edgedy of the mesh */
+  const double *inv_src_x, *inv_src_y;
+  int inv_nx, inv_ny;
   double* h_stage;      /* pinned: [0..1] dt pair, [2..5] sums */
   int* h_status;        /* pinned */
   int schedule;

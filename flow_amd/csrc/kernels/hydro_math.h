@@ -122,16 +122,18 @@ template <typename T>
 __device__ __forceinline__ T mc_upwind(T w0, T w1, T w2, T w3, T inv, T vel,
                                        bool upwind_low, T cd_lo, T cd_hi,
                                        T dt_h) {
-  if (upwind_low) {
-    const T a0 = T(0.5) * inv * (w2 - w0);
-    const T b0 = T(2.0) * inv * (w1 - w0);
-    const T c0 = T(2.0) * inv * (w2 - w1);
-    return w1 + T(0.5) * minmod(minmod(a0, b0), c0) * (cd_lo - vel * dt_h);
-  }
+  /* Both candidates are evaluated and one is selected: the selected value
+   * is the one the reference's branch computes, and the wave stays
+   * convergent with every stencil load issued up front. */
+  const T a0 = T(0.5) * inv * (w2 - w0);
+  const T b0 = T(2.0) * inv * (w1 - w0);
+  const T c0 = T(2.0) * inv * (w2 - w1);
+  const T lo = w1 + T(0.5) * minmod(minmod(a0, b0), c0) * (cd_lo - vel * dt_h);
   const T a1 = T(0.5) * inv * (w3 - w1);
   const T b1 = T(2.0) * inv * (w2 - w1);
   const T c1 = T(2.0) * inv * (w3 - w2);
-  return w2 - T(0.5) * minmod(minmod(a1, b1), c1) * (cd_hi + vel * dt_h);
+  const T hi = w2 - T(0.5) * minmod(minmod(a1, b1), c1) * (cd_hi + vel * dt_h);
+  return upwind_low ? lo : hi;
 }
 
 template <typename T>
@@ -145,27 +147,28 @@ struct FaceFlux {
  *   vm, v0, vp   face velocities at f-1, f, f+1
  *   rmm..rpp     densities of cells f-2, f-1, f, f+1
  *   emm..epp     specific energies of the same cells
- *   ed           face spacing at f; cd_lo, cd_hi cell spacings of f-1, f
+ *   inv          1.0 / face spacing at f (the reference's invdx, :344);
+ *                cd_lo, cd_hi cell spacings of f-1, f
  *   et           transverse edge length (edgedy[i] in x, edgedx[j] in y)
  * The y sweep selects the Van Leer smoothness by the face velocity v0
  * instead of the time-centred one (omp3/flow.c:477 vs :361). */
 template <typename T, bool kYSweep>
 __device__ __forceinline__ FaceFlux<T> mass_energy_flux(
     T vm, T v0, T vp, T rmm, T rm, T rp, T rpp, T emm, T em, T ep, T epp,
-    T ed, T cd_lo, T cd_hi, T et, T dt, T dt_h) {
-  const T inv = T(1.0) / ed;
+    T inv, T cd_lo, T cd_hi, T et, T dt, T dt_h) {
   const T s_c = T(0.5) * inv * (vp - vm);
   const T s_r = T(2.0) * inv * (v0 - vm);
   const T s_l = T(2.0) * inv * (vp - v0);
   const T v_tc = v0 - T(0.5) * v0 * dt * minmod(s_c, minmod(s_r, s_l));
 
-  T limiter = T(0.0);
+  /* Van Leer limiter. The smoothness numerator is selected before the one
+   * division; where drho == 0 the quotient is discarded and the limiter is
+   * 0, as in the reference's if (density_diff). */
   const T drho = (rp - rm);
-  if (drho != T(0.0)) {
-    const T sel = kYSweep ? v0 : v_tc;
-    const T r = (sel >= T(0.0)) ? (rm - rmm) / drho : (rpp - rp) / drho;
-    limiter = (r + vabs(r)) / (T(1.0) + vabs(r));
-  }
+  const T sel = kYSweep ? v0 : v_tc;
+  const T r = ((sel >= T(0.0)) ? (rm - rmm) : (rpp - rp)) / drho;
+  const T limiter =
+      (drho != T(0.0)) ? (r + vabs(r)) / (T(1.0) + vabs(r)) : T(0.0);
   const T rho_up = (v_tc >= T(0.0)) ? rm : rp;
   FaceFlux<T> out;
   out.mass = (v_tc * rho_up + T(0.5) * vabs(v_tc) *
@@ -198,12 +201,12 @@ __device__ __forceinline__ CellState<T> sweep_update(
 }
 
 /* Momentum flux = advected mass flux x interpolated velocity:
- * F = len * 0.5 * (Fa + Fb); flux = F * w_interp (omp3/flow.c:809-820 etc.). */
+ * F = len * 0.5 * (Fa + Fb); flux = F * w_interp (omp3/flow.c:809-820 etc.).
+ * inv is 1.0 / the spacing along the flux direction (invdx, :792). */
 template <typename T>
 __device__ __forceinline__ T momentum_flux(T len, T fa, T fb, T w0, T w1,
-                                           T w2, T w3, T ed, T vel, T cd_lo,
+                                           T w2, T w3, T inv, T vel, T cd_lo,
                                            T cd_hi, T dt_h) {
-  const T inv = T(1.0) / ed;
   const T Fc = len * T(0.5) * (fa + fb);
   const T w = mc_upwind(w0, w1, w2, w3, inv, vel, vel > T(0.0), cd_lo, cd_hi,
                         dt_h);

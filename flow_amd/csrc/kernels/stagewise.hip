@@ -48,6 +48,12 @@ __global__ void k_boundary(double* __restrict__ a, int nxp, int nyp, int pad,
   }
 }
 
+__global__ void k_reciprocal(int n, double* __restrict__ dst,
+                             const double* __restrict__ src) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = 1.0 / src[k];
+}
+
 __global__ void k_copy(long n, double* __restrict__ dst,
                        const double* __restrict__ src) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,7 +144,7 @@ __global__ void k_x_flux(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, c = i * nx + j, f = i * (nx + 1) + j;
   const FaceFlux<double> fl = mass_energy_flux<double, false>(
       u[f - 1], u[f], u[f + 1], rho[c - 2], rho[c - 1], rho[c], rho[c + 1],
-      e[c - 2], e[c - 1], e[c], e[c + 1], m.edgedx[j], m.celldx[j - 1],
+      e[c - 2], e[c - 1], e[c], e[c + 1], 1.0 / m.edgedx[j], m.celldx[j - 1],
       m.celldx[j], m.edgedy[i], dt, dt_h);
   F_x[f] = fl.mass;
   eF_x[f] = fl.energy;
@@ -154,7 +160,7 @@ __global__ void k_y_flux(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, c = i * nx + j;
   const FaceFlux<double> fl = mass_energy_flux<double, true>(
       v[c - nx], v[c], v[c + nx], rho[c - 2 * nx], rho[c - nx], rho[c],
-      rho[c + nx], e[c - 2 * nx], e[c - nx], e[c], e[c + nx], m.edgedy[i],
+      rho[c + nx], e[c - 2 * nx], e[c - nx], e[c], e[c + nx], 1.0 / m.edgedy[i],
       m.celldy[i - 1], m.celldy[i], m.edgedx[j], dt, dt_h);
   F_y[c] = fl.mass;
   eF_y[c] = fl.energy;
@@ -202,7 +208,7 @@ __global__ void k_mom_flux_xx(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, c = i * nx + j, f = i * (nx + 1) + j;
   const double u_cell = 0.5 * (u[f] + u[f + 1]);
   uF_x[c] = momentum_flux(m.edgedy[i], F_x[f], F_x[f + 1], u[f - 1], u[f],
-                          u[f + 1], u[f + 2], m.edgedx[j], u_cell,
+                          u[f + 1], u[f + 2], 1.0 / m.edgedx[j], u_cell,
                           m.celldx[j - 1], m.celldx[j], dt_h);
 }
 
@@ -217,7 +223,7 @@ __global__ void k_mom_flux_xy(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, sx = nx + 1, c = i * nx + j, f = i * sx + j;
   const double v_cell = 0.5 * (v[c - 1] + v[c]);
   uF_y[f] = momentum_flux(m.edgedx[j], F_y[c], F_y[c - 1], u[f - 2 * sx],
-                          u[f - sx], u[f], u[f + sx], m.edgedy[i], v_cell,
+                          u[f - sx], u[f], u[f + sx], 1.0 / m.edgedy[i], v_cell,
                           m.celldy[i - 1], m.celldy[i], dt_h);
 }
 
@@ -232,7 +238,7 @@ __global__ void k_mom_flux_yx(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, sx = nx + 1, c = i * nx + j, f = i * sx + j;
   const double u_cell = 0.5 * (u[f] + u[f - sx]);
   vF_x[f] = momentum_flux(m.celldy[i], F_x[f], F_x[f - sx], v[c - 2],
-                          v[c - 1], v[c], v[c + 1], m.edgedx[j], u_cell,
+                          v[c - 1], v[c], v[c + 1], 1.0 / m.edgedx[j], u_cell,
                           m.celldx[j - 1], m.celldx[j], dt_h);
 }
 
@@ -246,7 +252,7 @@ __global__ void k_mom_flux_yy(FkMesh m, Domain d, const double* dtp,
   const int nx = m.nx, c = i * nx + j;
   const double v_cell = 0.5 * (v[c] + v[c + nx]);
   vF_y[c] = momentum_flux(m.celldx[j], F_y[c], F_y[c + nx], v[c - nx], v[c],
-                          v[c + nx], v[c + 2 * nx], m.edgedy[i], v_cell,
+                          v[c + nx], v[c + 2 * nx], 1.0 / m.edgedy[i], v_cell,
                           m.celldy[i - 1], m.celldy[i], dt_h);
 }
 
@@ -331,6 +337,14 @@ hipError_t fk_boundary(int nxp, int nyp, int pad, double* arr, int invert,
   const double sy = (invert == 2) ? -1.0 : 1.0;
   hipLaunchKernelGGL(k_boundary, dim3((n + 255) / 256), dim3(256), 0, s, arr,
                      nxp, nyp, pad, sx, sy);
+  return hipGetLastError();
+}
+
+hipError_t fk_reciprocal(int n, double* dst, const double* src,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reciprocal, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                     dst, src);
   return hipGetLastError();
 }
 

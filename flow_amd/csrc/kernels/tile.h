@@ -34,6 +34,17 @@ __device__ __forceinline__ bool is_ghost(int k, int n) {
   return k < kPad || k >= n - kPad;
 }
 
+/* Mirror / ghost tests specialised on whether the tile touches the domain
+ * edge: an interior tile's halo is all interior. */
+template <bool EDGE>
+__device__ __forceinline__ int mir(int k, int n) {
+  return EDGE ? mirror(k, n) : k;
+}
+template <bool EDGE>
+__device__ __forceinline__ bool ghost(int k, int n) {
+  return EDGE ? is_ghost(k, n) : false;
+}
+
 struct TileList {
   int ntx, nty;  /* tiles along x and y */
   int ntiles;
@@ -76,8 +87,12 @@ struct TileWalk {
  * block's NT threads. */
 template <int NT, int ROWS, int COLS, class F>
 __device__ __forceinline__ void region(F&& f) {
-#pragma unroll 1
-  for (int k = threadIdx.x; k < ROWS * COLS; k += NT) f(k / COLS, k % COLS);
+  constexpr int kIters = (ROWS * COLS + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < kIters; ++it) {
+    const int k = threadIdx.x + it * NT;
+    if (k < ROWS * COLS) f(k / COLS, k % COLS);
+  }
 }
 
 inline int persistent_grid(int ntiles, int blocks_per_cu) {

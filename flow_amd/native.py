@@ -7,7 +7,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "lib", "libflowhip.so")
+_LIB = os.environ.get("FLOWHIP_LIBRARY") or os.path.join(_HERE, "lib", "libflowhip.so")
 
 STAGEWISE, FUSED = 0, 1
 NO_INVERT, INVERT_X, INVERT_Y = 0, 1, 2

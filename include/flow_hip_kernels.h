@@ -23,6 +23,10 @@ extern "C" {
 typedef struct FkMesh {
   int nx, ny, pad;
   const double *edgedx, *edgedy, *celldx, *celldy; /* device, L+1 each */
+  /* 1.0 / edgedx and 1.0 / edgedy (the reference's invdx / invdy,
+   * omp3/flow.c:344, 460), computed once per mesh by fk_reciprocal; read
+   * by the fused kernels only. */
+  const double *inv_edgedx, *inv_edgedy;
 } FkMesh;
 
 /* Device scratch for the single-pass grid reductions. */
@@ -43,6 +47,9 @@ int fk_reduce_blocks(void);
 hipError_t fk_boundary(int nxp, int nyp, int pad, double* arr, int invert,
                        hipStream_t s);
 hipError_t fk_copy(long n, double* dst, const double* src, hipStream_t s);
+/* dst[k] = 1.0 / src[k] for k < n (correctly rounded division). */
+hipError_t fk_reciprocal(int n, double* dst, const double* src,
+                         hipStream_t s);
 
 /* ---- stagewise schedule: one kernel per omp3 loop ---- */
 /* omp3/flow.c:59-67 */
