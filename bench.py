@@ -57,7 +57,8 @@ def parse_args():
     p.add_argument("--dt", type=float, default=1.0e-6,
                    help=".params dt (the first step uses C_T * dt)")
     p.add_argument("--seed", type=int, default=20250101)
-    p.add_argument("--schedule", choices=("fused", "stagewise"), default=None)
+    p.add_argument("--schedule", choices=("fused", "march", "stagewise"), default=None)
+    p.add_argument("--march-rows", type=int, default=None)
     p.add_argument("--graphs", action="store_true")
     p.add_argument("--timing-steps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -135,7 +136,10 @@ def main():
     mesh = flow_amd.Mesh(n, n, dt=args.dt)
     fused_ok = hasattr(native.lib(), "fk_lagrangian")
     sched_name = args.schedule or ("fused" if fused_ok else "stagewise")
-    mesh.set_schedule(flow_amd.FUSED if sched_name == "fused" else flow_amd.STAGEWISE)
+    mesh.set_schedule({"fused": flow_amd.FUSED, "march": flow_amd.MARCH,
+                       "stagewise": flow_amd.STAGEWISE}[sched_name])
+    if args.march_rows:
+        mesh.set_march_rows(args.march_rows)
     mesh.set_graphs(args.graphs)
     st = flow_amd.DeviceState(mesh)
 

@@ -32,6 +32,7 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   memset(h->h_stage, 0, sizeof(double) * 8);
   memset(h->h_status, 0, sizeof(int) * 4);
   h->schedule = FLOWHIP_FUSED;
+  h->march_rows = 64;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   return h;

@@ -40,6 +40,7 @@ typedef struct FlowHipCtx {
   double* h_stage;      /* pinned: [0..1] dt pair, [2..5] sums */
   int* h_status;        /* pinned */
   int schedule;
+  int march_rows;
   int use_graphs;
   hipGraphExec_t graph[3]; /* even, odd, first step */
   const void* graph_key[3][16];

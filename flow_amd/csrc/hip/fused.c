@@ -24,9 +24,14 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
   FK_LAUNCH(h, "lagrangian",
             fk_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1, my1, u0,
                           v0, e1, rho0, h->red, st->max_dt, st->tt == 0, s));
-  FK_LAUNCH(h, "advect_mass_energy",
-            fk_advect_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho, st->e,
-                                  st->F_x, st->F_y, h->red, s));
+  if (h->schedule == FLOWHIP_MARCH)
+    FK_LAUNCH(h, "advect_mass_energy",
+              fk_march_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho, st->e,
+                                   st->F_x, st->F_y, h->red, h->march_rows, s));
+  else
+    FK_LAUNCH(h, "advect_mass_energy",
+              fk_advect_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho,
+                                    st->e, st->F_x, st->F_y, h->red, s));
   FK_LAUNCH(h, "advect_momentum_x",
             fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,
                                  mx1, st->mom_x, st->u, h->red.dtp, s));

@@ -11,12 +11,20 @@
 #include "schedules.h"
 
 int flowhip_set_schedule(Mesh* mesh, int schedule) {
-  if (schedule != FLOWHIP_STAGEWISE && schedule != FLOWHIP_FUSED) return -1;
+  if (schedule != FLOWHIP_STAGEWISE && schedule != FLOWHIP_FUSED &&
+      schedule != FLOWHIP_MARCH)
+    return -1;
   flowhip_ctx(mesh)->schedule = schedule;
   return 0;
 }
 
 int flowhip_get_schedule(Mesh* mesh) { return flowhip_ctx(mesh)->schedule; }
+
+int flowhip_set_march_rows(Mesh* mesh, int rows) {
+  if (rows < 4 || rows > 120) return -1;
+  flowhip_ctx(mesh)->march_rows = rows;
+  return 0;
+}
 
 int flowhip_set_graphs(Mesh* mesh, int on) {
   flowhip_ctx(mesh)->use_graphs = on ? 1 : 0;

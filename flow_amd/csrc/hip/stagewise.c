@@ -127,7 +127,7 @@ void flowhip_step_stagewise(FlowHipCtx* h, const FlowHipStep* st) {
 }
 
 void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {
-  if (h->schedule == FLOWHIP_FUSED)
+  if (h->schedule == FLOWHIP_FUSED || h->schedule == FLOWHIP_MARCH)
     flowhip_step_fused(h, st);
   else
     flowhip_step_stagewise(h, st);

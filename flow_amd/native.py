@@ -9,7 +9,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.environ.get("FLOWHIP_LIBRARY") or os.path.join(_HERE, "lib", "libflowhip.so")
 
-STAGEWISE, FUSED = 0, 1
+STAGEWISE, FUSED, MARCH = 0, 1, 2
 NO_INVERT, INVERT_X, INVERT_Y = 0, 1, 2
 RECV, SEND = 0, 1
 
@@ -61,6 +61,7 @@ def lib():
         "flowhip_set_schedule": (i, [mp, i]),
         "flowhip_get_schedule": (i, [mp]),
         "flowhip_set_graphs": (i, [mp, i]),
+        "flowhip_set_march_rows": (i, [mp, i]),
         "flowhip_set_timing": (i, [mp, i]),
         "flowhip_timing_count": (i, [mp]),
         "flowhip_timing_entry": (i, [mp, i, ctypes.c_char_p, i, _dp,
@@ -135,6 +136,10 @@ class Mesh:
     def set_schedule(self, schedule):
         if lib().flowhip_set_schedule(self.ptr, schedule) != 0:
             raise ValueError(f"bad schedule {schedule}")
+
+    def set_march_rows(self, rows):
+        if lib().flowhip_set_march_rows(self.ptr, rows) != 0:
+            raise ValueError(f"bad chunk height {rows}")
 
     def set_graphs(self, on):
         lib().flowhip_set_graphs(self.ptr, 1 if on else 0)

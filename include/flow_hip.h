@@ -24,9 +24,12 @@ extern "C" {
 #define FLOWHIP_FUSED 1     /* fused Lagrangian, advection and momentum
                                kernels; only the carried state and the
                                fluxes the momentum step reads are written */
+#define FLOWHIP_MARCH 2     /* the fused schedule with row-marching kernels */
 
 int flowhip_set_schedule(Mesh* mesh, int schedule);
 int flowhip_get_schedule(Mesh* mesh);
+/* Rows per chunk of the row-marching kernels: 4..120, default 64. */
+int flowhip_set_march_rows(Mesh* mesh, int rows);
 /* Replay each step from a captured hipGraph (one per sweep-order parity and
  * for the first step). */
 int flowhip_set_graphs(Mesh* mesh, int on);

@@ -158,6 +158,15 @@ hipError_t fk_advect_momentum_y(FkMesh m, int tt, const double* u1,
                                 const double* my1, double* mom_y, double* v,
                                 const double* dtp, hipStream_t s);
 
+/* ---- row-marching forms (march.hip) ----
+ * Same contracts as the fused kernels above; one wave marches a strip of
+ * columns down chunks of chunk_rows rows. */
+hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
+                                const double* v0, const double* rho0,
+                                const double* e1, double* rho, double* e,
+                                double* F_x, double* F_y, FkReduce r,
+                                int chunk_rows, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,10 +23,12 @@ def oracle_run(problem_fn, n, steps, width=1.0, max_dt=1.0e30, dt=1.0e-4,
 
 
 def gpu_run(problem_fn, n, steps, schedule, width=1.0, max_dt=1.0e30,
-            dt=1.0e-4, graphs=False, init=None):
+            dt=1.0e-4, graphs=False, init=None, march_rows=None):
     import flow_amd
     m = flow_amd.Mesh(n, n, width=width, height=width, max_dt=max_dt, dt=dt)
     m.set_schedule(schedule)
+    if march_rows:
+        m.set_march_rows(march_rows)
     m.set_graphs(graphs)
     st = flow_amd.DeviceState(m)
     rho, e = init if init is not None else problem_fn(n)

@@ -88,12 +88,14 @@ FUSED_CASES = CASES + [
 ]
 
 
+@pytest.mark.parametrize("schedule", ["fused", "march"])
 @pytest.mark.parametrize("name,n,steps,dt", FUSED_CASES)
-def test_fused_matches_oracle_bitwise(name, n, steps, dt):
-    print(f"case fused {name} n={n} steps={steps} dt={dt}", flush=True)
+def test_fused_matches_oracle_bitwise(name, n, steps, dt, schedule):
+    print(f"case {schedule} {name} n={n} steps={steps} dt={dt}", flush=True)
     fn = problems.PROBLEMS[name]
     ref = harness.oracle_run(fn, n, steps, dt=dt)
-    got = harness.gpu_run(fn, n, steps, flow_amd.FUSED, dt=dt)
+    sched = flow_amd.FUSED if schedule == "fused" else flow_amd.MARCH
+    got = harness.gpu_run(fn, n, steps, sched, dt=dt, march_rows=16)
     _check(got, ref, ref["L"], bitwise=True)
     L = ref["L"]
     for k in ("F_x", "F_y", "rho_old"):
@@ -109,25 +111,29 @@ def test_fused_matches_oracle_bitwise(name, n, steps, dt):
         assert diff == 0, (k, diff)
 
 
-def test_fused_is_deterministic():
-    print("case fused determinism n=300", flush=True)
-    a = harness.gpu_run(problems.multi_blast, 300, 4, flow_amd.FUSED, dt=1e-5)
-    b = harness.gpu_run(problems.multi_blast, 300, 4, flow_amd.FUSED, dt=1e-5)
+@pytest.mark.parametrize("schedule", ["fused", "march"])
+def test_fused_is_deterministic(schedule):
+    print(f"case {schedule} determinism n=300", flush=True)
+    sched = flow_amd.FUSED if schedule == "fused" else flow_amd.MARCH
+    a = harness.gpu_run(problems.multi_blast, 300, 4, sched, dt=1e-5)
+    b = harness.gpu_run(problems.multi_blast, 300, 4, sched, dt=1e-5)
     for k in flow_amd.ARRAY_NAMES:
         assert np.array_equal(a["arrays"][k], b["arrays"][k], equal_nan=True), k
     np.testing.assert_array_equal(a["mass"], b["mass"])
     np.testing.assert_array_equal(a["energy"], b["energy"])
 
 
-def test_fused_kat_flow_params():
-    """flow.tests:1 through the fused schedule."""
+@pytest.mark.parametrize("schedule", ["fused", "march"])
+def test_fused_kat_flow_params(schedule):
+    """flow.tests:1 through the fused schedules."""
     import json, os
     here = os.path.dirname(os.path.abspath(__file__))
     kat = json.load(open(os.path.join(here, "golden", "flow_tests_kat.json")))
     n = kat["nx"]
-    print(f"case fused KAT n={n}", flush=True)
+    print(f"case {schedule} KAT n={n}", flush=True)
+    sched = flow_amd.FUSED if schedule == "fused" else flow_amd.MARCH
     got = harness.gpu_run(problems.flow_params_problem, n, kat["iterations"],
-                          flow_amd.FUSED, dt=kat["dt"])
+                          sched, dt=kat["dt"])
     L = got["L"]
     v = harness.views(got["arrays"], L)
     dsum = float(v["rho"][2:L - 2, 2:L - 2].sum())
