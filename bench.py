@@ -135,7 +135,7 @@ def main():
     rho0, e0 = problems.multi_blast(n, seed=args.seed)
     mesh = flow_amd.Mesh(n, n, dt=args.dt)
     fused_ok = hasattr(native.lib(), "fk_lagrangian")
-    sched_name = args.schedule or ("fused" if fused_ok else "stagewise")
+    sched_name = args.schedule or ("march" if fused_ok else "stagewise")
     mesh.set_schedule({"fused": flow_amd.FUSED, "march": flow_amd.MARCH,
                        "stagewise": flow_amd.STAGEWISE}[sched_name])
     if args.march_rows:

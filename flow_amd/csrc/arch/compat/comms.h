@@ -1,0 +1,4 @@
+/* comms.h -- arch's comms.h as seen by flow's sources (included as "../comms.h" from
+ * the flow directory): every declaration the hip kernel set provides in its
+ * place is in flow_arch.h. */
+#include "flow_arch.h"

@@ -1,0 +1,4 @@
+/* params.h -- arch's params.h as seen by flow's sources (included as "../params.h" from
+ * the flow directory): every declaration the hip kernel set provides in its
+ * place is in flow_arch.h. */
+#include "flow_arch.h"

@@ -6,6 +6,31 @@
 
 #include "flowhip_ctx.h"
 
+/* FLOWHIP_SCHEDULE (stagewise | fused | march) and FLOWHIP_MARCH_ROWS pick
+ * the schedule for callers that only see flow_interface.h, such as flow's
+ * own driver. */
+static void schedule_from_env(FlowHipCtx* h) {
+  const char* s = getenv("FLOWHIP_SCHEDULE");
+  if (s && *s) {
+    if (!strcmp(s, "stagewise"))
+      h->schedule = FLOWHIP_STAGEWISE;
+    else if (!strcmp(s, "fused"))
+      h->schedule = FLOWHIP_FUSED;
+    else if (!strcmp(s, "march"))
+      h->schedule = FLOWHIP_MARCH;
+    else
+      TERMINATE("FLOWHIP_SCHEDULE must be stagewise, fused or march, not '%s'", s);
+  }
+  const char* r = getenv("FLOWHIP_MARCH_ROWS");
+  if (r && *r) {
+    char* end = NULL;
+    const long rows = strtol(r, &end, 10);
+    if (*end != '\0' || rows < 4 || rows > 120)
+      TERMINATE("FLOWHIP_MARCH_ROWS must be an integer in [4, 120], not '%s'", r);
+    h->march_rows = (int)rows;
+  }
+}
+
 FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   if (mesh->hip) return (FlowHipCtx*)mesh->hip;
   FlowHipCtx* h = (FlowHipCtx*)calloc(1, sizeof(FlowHipCtx));
@@ -31,8 +56,9 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
                          hipHostMallocDefault));
   memset(h->h_stage, 0, sizeof(double) * 8);
   memset(h->h_status, 0, sizeof(int) * 4);
-  h->schedule = FLOWHIP_FUSED;
-  h->march_rows = 64;
+  h->schedule = FLOWHIP_MARCH;
+  h->march_rows = 32;
+  schedule_from_env(h);
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   return h;

@@ -32,12 +32,23 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
     FK_LAUNCH(h, "advect_mass_energy",
               fk_advect_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho,
                                     st->e, st->F_x, st->F_y, h->red, s));
-  FK_LAUNCH(h, "advect_momentum_x",
-            fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,
-                                 mx1, st->mom_x, st->u, h->red.dtp, s));
-  FK_LAUNCH(h, "advect_momentum_y",
-            fk_advect_momentum_y(m, st->tt, st->u, v0, st->F_x, st->F_y,
-                                 st->rho, my1, st->mom_y, st->v, h->red.dtp, s));
+  if (h->schedule == FLOWHIP_MARCH) {
+    FK_LAUNCH(h, "advect_momentum_x",
+              fk_march_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,
+                                  mx1, st->mom_x, st->u, h->red.dtp,
+                                  h->march_rows, s));
+    FK_LAUNCH(h, "advect_momentum_y",
+              fk_march_momentum_y(m, st->tt, st->u, v0, st->F_x, st->F_y,
+                                  st->rho, my1, st->mom_y, st->v, h->red.dtp,
+                                  h->march_rows, s));
+  } else {
+    FK_LAUNCH(h, "advect_momentum_x",
+              fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,
+                                   mx1, st->mom_x, st->u, h->red.dtp, s));
+    FK_LAUNCH(h, "advect_momentum_y",
+              fk_advect_momentum_y(m, st->tt, st->u, v0, st->F_x, st->F_y,
+                                   st->rho, my1, st->mom_y, st->v, h->red.dtp, s));
+  }
   /* the advection kernel left the totals of the new rho, e in h->red.sums */
   h->sums_valid = 1;
   h->sums_rho = st->rho;

@@ -92,8 +92,9 @@ struct RowGeo {
   T a, b;
   int base;
   __device__ RowGeo(const T* arr, int base_row, int last_row, int lane) : base(base_row) {
-    a = arr[min(base_row + lane, last_row)];
-    b = arr[min(base_row + kWave + lane, last_row)];
+    /* rows outside [0, last_row] are never asked for; clamp the loads */
+    a = arr[max(0, min(base_row + lane, last_row))];
+    b = arr[max(0, min(base_row + kWave + lane, last_row))];
   }
   /* value at row r (uniform, base <= r < base + 128) */
   __device__ __forceinline__ T operator()(int r) const {
@@ -410,6 +411,436 @@ __global__ __launch_bounds__(kMNT) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io
 }
 
 
+/* ======================================================================= */
+/* momentum                                                                */
+/* ======================================================================= */
+
+template <typename T>
+struct MMomIO {
+  const T* __restrict__ u;    /* x velocity read (u0 for x, u1 for y) */
+  const T* __restrict__ v0;   /* y velocity after the viscous step */
+  const T* __restrict__ F_x;
+  const T* __restrict__ F_y;
+  const T* __restrict__ rho;  /* advected density */
+  const T* __restrict__ mom1; /* momentum after the viscous step */
+  T* __restrict__ mom;        /* advected momentum */
+  T* __restrict__ vel;        /* recomputed velocity, + ghosts */
+};
+
+/* value of v at lane src of the wave */
+template <typename T>
+__device__ __forceinline__ T from_lane(T v, int src) {
+  return __shfl(v, src, kWave);
+}
+
+/* lane holding the mirror of index q (lane holds q), kept inside the wave */
+__device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
+  return min(max(lane + (mirror(q, n) - q), 0), kWave - 1);
+}
+
+/* x momentum, odd tt (omp3/flow.c:570-619): x flux in x, u recompute, then
+ * x flux in y. Lane l holds x face / cell column q = x0 - 2 + l. Per row k
+ * the cell fluxes, the first sub-update and u1 are computed two rows ahead
+ * of the output row r, which takes the corner fluxes at rows r and r+1 from
+ * a window of u1 rows r-1 .. r+2. Ghost rows are the mirror row's values
+ * (u is copied unsigned across the south/north walls). */
+template <typename T>
+__global__ __launch_bounds__(kMNT) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
+                                                        const double* dtp,
+                                                        MarchList ml) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const T dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves) {
+    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                      (y1 + 3 > ny - kPad);
+    const int q = x0 - 2 + lane;
+    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
+    const bool cell_ghost = is_ghost(q, nx);
+    const int mc_lane = mirror_lane(lane, q, nx);
+    const bool own = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
+    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+    const T iedx_q = g.iedx[qg];
+    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+
+    /* u1 and m1 of row i from that row's u0, F_x, mom1 and rho */
+    auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
+      const T up1 = dn(a_u, 1);
+      T ufx = momentum_flux(edy(i), a_fx, dn(a_fx, 1), up(a_u, 1), a_u, up1,
+                            dn(a_u, 2), iedx_q, T(0.5) * (a_u + up1), cdx_qm,
+                            cdx_q, dt_h);
+      const T ufx_m = from_lane(ufx, mc_lane);
+      if (cell_ghost) ufx = ufx_m;
+      m1 = advect_update(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i));
+      const T rex = rho_edge_x(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q);
+      u1 = velocity(m1, rex);
+    };
+    /* x flux in y at corner row gf from u1 rows gf-2 .. gf+1 */
+    auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
+      const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
+      return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
+                           v_cell, cdy(gf - 1), cdy(gf), dt_h);
+    };
+
+    T wu[4], wm[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = mirror(y0 - 2 + k, ny);
+      T m1, u1;
+      xrow(i, io.u[i * sx + qf], io.F_x[i * sx + qf], io.mom1[i * sx + qf],
+           io.rho[i * nx + qc], m1, u1);
+      wu[k] = u1;
+      if (k >= 2) wm[k - 2] = m1;
+    }
+    T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], io.v0[y0 * nx + qc],
+                       io.F_y[y0 * nx + qc]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
+    int ia = mirror(y0 + 2, ny);
+    T a_u = io.u[ia * sx + qf], a_fx = io.F_x[ia * sx + qf];
+    T a_m1 = io.mom1[ia * sx + qf], a_r = io.rho[ia * nx + qc];
+    T a_v = io.v0[(y0 + 1) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
+    for (int r = y0; r < y1; ++r) {
+      const T c_u = a_u, c_fx = a_fx, c_m1 = a_m1, c_r = a_r, c_v = a_v, c_fy = a_fy;
+      {
+        const int in = mirror(r + 3, ny), rv = min(r + 2, ny);
+        a_u = io.u[in * sx + qf];
+        a_fx = io.F_x[in * sx + qf];
+        a_m1 = io.mom1[in * sx + qf];
+        a_r = io.rho[in * nx + qc];
+        a_v = io.v0[rv * nx + qc];
+        a_fy = io.F_y[rv * nx + qc];
+      }
+      T m1, u1;
+      xrow(mirror(r + 2, ny), c_u, c_fx, c_m1, c_r, m1, u1);
+      wu[3] = u1;
+      wm[2] = m1;
+      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
+      if (own) {
+        io.mom[r * sx + q] = advect_update(wm[0], dt_h, uf, uf_prev, cdx_q, edy(r));
+        mput_xface(edge, io.vel, nx, ny, r, q, wu[1], T(-1.0));
+      }
+      uf_prev = uf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
+      wm[0] = wm[1];
+      wm[1] = wm[2];
+    }
+  }
+}
+
+/* x momentum, even tt (omp3/flow.c:673-722): x flux in y, u recompute, then
+ * x flux in x. Lane l holds x face / cell column q = x0 - 3 + l (strips of
+ * 59 columns, so faces [x0-2, x1+1] and the cells between them fit the
+ * wave). Output row r takes the corner fluxes at rows r and r+1, the second
+ * from a window of u0 rows r-1 .. r+2; ghost faces hold the negated mirror
+ * face's u1 (INVERT_X) and ghost cells the mirror cell's flux. */
+template <typename T>
+__global__ __launch_bounds__(kMNT) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
+                                                        const double* dtp,
+                                                        MarchList ml) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const T dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves) {
+    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                      (y1 + 3 > ny - kPad);
+    const int q = x0 - 3 + lane;
+    const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
+    const int qg = min(max(q, 1), nx);
+    const bool face_ghost = is_ghost(q, sx), cell_ghost = is_ghost(q, nx);
+    const int mf_lane = mirror_lane(lane, q, sx), mc_lane = mirror_lane(lane, q, nx);
+    const bool own = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
+    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+    const T iedx_q = g.iedx[qg];
+    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+
+    auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
+      const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
+      return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
+                           v_cell, cdy(gf - 1), cdy(gf), dt_h);
+    };
+
+    /* u0 rows r-1 .. r+2 of face q (ghost rows are in memory) */
+    T wu[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wu[k] = io.u[(y0 - 2 + k) * sx + qf];
+    T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], io.v0[y0 * nx + qc],
+                       io.F_y[y0 * nx + qc]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
+    T a_u = io.u[(y0 + 2) * sx + qf];
+    T a_v = io.v0[(y0 + 1) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
+    T a_m1 = io.mom1[y0 * sx + qf], a_r = io.rho[y0 * nx + qc];
+    T a_fx = io.F_x[y0 * sx + qf];
+    for (int r = y0; r < y1; ++r) {
+      wu[3] = a_u;
+      const T c_v = a_v, c_fy = a_fy, c_m1 = a_m1, c_r = a_r, c_fx = a_fx;
+      {
+        const int ru = min(r + 3, ny - 1), rv = min(r + 2, ny), rc = min(r + 1, ny - 1);
+        a_u = io.u[ru * sx + qf];
+        a_v = io.v0[rv * nx + qc];
+        a_fy = io.F_y[rv * nx + qc];
+        a_m1 = io.mom1[rc * sx + qf];
+        a_r = io.rho[rc * nx + qc];
+        a_fx = io.F_x[rc * sx + qf];
+      }
+      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
+      const T m1 = advect_update(c_m1, dt_h, uf, uf_prev, cdx_q, edy(r));
+      const T rex = rho_edge_x(c_r, up(c_r, 1), cdx_q, cdx_qm, cdy(r), edx_q);
+      T u1 = velocity(m1, rex);
+      const T u1_m = from_lane(u1, mf_lane);
+      if (face_ghost) u1 = -u1_m;
+      const T up1 = dn(u1, 1);
+      T ufx = momentum_flux(edy(r), c_fx, dn(c_fx, 1), up(u1, 1), u1, up1, dn(u1, 2),
+                            iedx_q, T(0.5) * (u1 + up1), cdx_qm, cdx_q, dt_h);
+      const T ufx_m = from_lane(ufx, mc_lane);
+      if (cell_ghost) ufx = ufx_m;
+      const T ufx_lo = up(ufx, 1);
+      if (own) {
+        io.mom[r * sx + q] = advect_update(m1, dt_h, ufx, ufx_lo, edx_q, cdy(r));
+        mput_xface(edge, io.vel, nx, ny, r, q, u1, T(-1.0));
+      }
+      uf_prev = uf;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
+    }
+  }
+}
+
+/* y momentum, odd tt (omp3/flow.c:621-669): y flux in x, v recompute, then
+ * y flux in y. Lane l holds column q = x0 - 2 + l. Per y-face row k (two
+ * rows ahead of the output row g) the corner fluxes along the row, the first
+ * sub-update and v1; ghost y faces past the north wall are the negated
+ * mirror (INVERT_Y). The cell fluxes at rows g-1 and g come from a window
+ * of v1 rows g-1 .. g+2; a ghost cell row's flux is its mirror row's. io.u
+ * is u after the x sub-updates. */
+template <typename T>
+__global__ __launch_bounds__(kMNT) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
+                                                        const double* dtp,
+                                                        MarchList ml) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const T dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves) {
+    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                      (y1 + 3 > ny - kPad);
+    const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
+    const int q = x0 - 2 + lane;
+    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
+    const bool own = q >= x0 && q < x1;
+    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+    const T iedx_q = g.iedx[qg];
+    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+
+    /* v1 and m1 of interior y-face row k from row k's v0, mom1 and rows
+     * k, k-1 of F_x, u and rho */
+    auto yrow = [&](int k, T a_v, T a_m1, T a_fx, T p_fx, T a_u, T p_u, T a_r,
+                    T p_r, T& m1, T& v1) {
+      const T vm1 = up(a_v, 1);
+      const T vfx = momentum_flux(cdy(k), a_fx, p_fx, up(a_v, 2), vm1, a_v,
+                                  dn(a_v, 1), iedx_q, T(0.5) * (a_u + p_u),
+                                  cdx_qm, cdx_q, dt_h);
+      m1 = advect_update(a_m1, dt_h, dn(vfx, 1), vfx, edx_q, cdy(k));
+      const T rey = rho_edge_y(a_r, p_r, cdx_q, cdy(k), cdy(k - 1), edy(k));
+      v1 = velocity(m1, rey);
+    };
+    /* y flux in y at cell row i from v1 rows i-1 .. i+2 */
+    auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
+      return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
+                           T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
+    };
+
+    T wv[4], wm[3];
+    wm[0] = wm[1] = wm[2] = T(0);
+    /* rows y0-2 .. y0+1; below the south wall, the negated mirror rows */
+    T p_fx = T(0), p_u = T(0), p_r = T(0);
+    {
+      const int k0 = max(y0 - 2, kPad);
+      p_fx = io.F_x[(k0 - 1) * sx + qf];
+      p_u = io.u[(k0 - 1) * sx + qf];
+      p_r = io.rho[(k0 - 1) * nx + qc];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = y0 - 2 + s;
+        if (k < kPad) continue;
+        const T c_fx = io.F_x[k * sx + qf], c_u = io.u[k * sx + qf];
+        const T c_r = io.rho[k * nx + qc];
+        T m1, v1;
+        yrow(k, io.v0[k * nx + qc], io.mom1[k * nx + qc], c_fx, p_fx, c_u, p_u,
+             c_r, p_r, m1, v1);
+        wv[s] = v1;
+        if (s >= 2) wm[s - 2] = m1;
+        p_fx = c_fx;
+        p_u = c_u;
+        p_r = c_r;
+      }
+    }
+    const bool south = (y0 == kPad);
+    if (south) {
+      wv[0] = -wv[3];
+      wv[1] = -wv[2];
+    }
+    T p_fy = io.F_y[y0 * nx + qc];
+    T vfy_prev = south ? T(0)
+                       : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
+                                  io.F_y[(y0 - 1) * nx + qc], p_fy);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
+    /* lookahead row y0+2 and the F_y row g+1 */
+    int kn = min(y0 + 2, ny - 1);
+    T a_v = io.v0[(y0 + 2) * nx + qc], a_m1 = io.mom1[(y0 + 2) * nx + qc];
+    T a_fx = io.F_x[kn * sx + qf], a_u = io.u[kn * sx + qf], a_r = io.rho[kn * nx + qc];
+    T a_fy = io.F_y[(y0 + 1) * nx + qc];
+    for (int gr = y0; gr < gend; ++gr) {
+      const int k = gr + 2;
+      const T c_v = a_v, c_m1 = a_m1, c_fx = a_fx, c_u = a_u, c_r = a_r, c_fy = a_fy;
+      {
+        const int ky = min(k + 1, ny), kx = min(k + 1, ny - 1);
+        a_v = io.v0[ky * nx + qc];
+        a_m1 = io.mom1[ky * nx + qc];
+        a_fx = io.F_x[kx * sx + qf];
+        a_u = io.u[kx * sx + qf];
+        a_r = io.rho[kx * nx + qc];
+        a_fy = io.F_y[min(gr + 2, ny) * nx + qc];
+      }
+      T m1 = T(0), v1;
+      if (k < ny - 1) {
+        yrow(k, c_v, c_m1, c_fx, p_fx, c_u, p_u, c_r, p_r, m1, v1);
+      } else {
+        /* ghost y face: mirror of face ny-1 is ny-2 (k-1), of ny is ny-3 (k-3) */
+        v1 = -((k == ny - 1) ? wv[2] : wv[0]);
+      }
+      wv[3] = v1;
+      wm[2] = m1;
+      T vfy = (gr >= ny - kPad)
+                  ? vfy_prev
+                  : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
+      if (south && gr == y0) vfy_prev = vfy;
+      if (own) {
+        io.mom[gr * nx + q] = advect_update(wm[0], dt_h, vfy, vfy_prev, cdx_q, edy(gr));
+        mput_yface(edge, io.vel, nx, ny, gr, q, wv[1], T(-1.0));
+      }
+      vfy_prev = vfy;
+      p_fx = c_fx;
+      p_u = c_u;
+      p_r = c_r;
+      p_fy = c_fy;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
+      wm[0] = wm[1];
+      wm[1] = wm[2];
+    }
+  }
+}
+
+/* y momentum, even tt (omp3/flow.c:724-774): y flux in y, v recompute, then
+ * y flux in x. Lane l holds column q = x0 - 2 + l. Output y-face row g takes
+ * the cell fluxes at rows g-1 and g (the second from a window of v0 rows
+ * g-1 .. g+2; a ghost cell row's flux is its mirror row's), then v1 along the
+ * row (ghost columns: the mirror column, unsigned) and the corner fluxes
+ * across the lanes. io.u is u after the x sub-updates. */
+template <typename T>
+__global__ __launch_bounds__(kMNT) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
+                                                        const double* dtp,
+                                                        MarchList ml) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const T dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves) {
+    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                      (y1 + 3 > ny - kPad);
+    const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
+    const int q = x0 - 2 + lane;
+    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
+    const bool col_ghost = is_ghost(q, nx);
+    const int mc_lane = mirror_lane(lane, q, nx);
+    const bool own = q >= x0 && q < x1;
+    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+    const T iedx_q = g.iedx[qg];
+    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+
+    auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
+      return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
+                           T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
+    };
+
+    /* v0 rows g-1 .. g+2 (ghost rows are in memory) */
+    T wv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) wv[s] = io.v0[(y0 - 2 + s) * nx + qc];
+    T p_fy = io.F_y[y0 * nx + qc];
+    const bool south = (y0 == kPad);
+    T vfy_prev = south ? T(0)
+                       : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
+                                  io.F_y[(y0 - 1) * nx + qc], p_fy);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
+    T p_fx = io.F_x[(y0 - 1) * sx + qf], p_u = io.u[(y0 - 1) * sx + qf];
+    T p_r = io.rho[(y0 - 1) * nx + qc];
+    T a_v = io.v0[(y0 + 2) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
+    T a_fx = io.F_x[y0 * sx + qf], a_u = io.u[y0 * sx + qf];
+    T a_r = io.rho[y0 * nx + qc], a_m1 = io.mom1[y0 * nx + qc];
+    for (int gr = y0; gr < gend; ++gr) {
+      wv[3] = a_v;
+      const T c_fy = a_fy, c_fx = a_fx, c_u = a_u, c_r = a_r, c_m1 = a_m1;
+      {
+        const int rx = min(gr + 1, ny - 1);
+        a_v = io.v0[min(gr + 3, ny) * nx + qc];
+        a_fy = io.F_y[min(gr + 2, ny) * nx + qc];
+        a_fx = io.F_x[rx * sx + qf];
+        a_u = io.u[rx * sx + qf];
+        a_r = io.rho[rx * nx + qc];
+        a_m1 = io.mom1[min(gr + 1, ny) * nx + qc];
+      }
+      T vfy = (gr >= ny - kPad)
+                  ? vfy_prev
+                  : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
+      if (south && gr == y0) vfy_prev = vfy;
+      const T m1 = advect_update(c_m1, dt_h, vfy, vfy_prev, cdx_q, edy(gr));
+      const T rey = rho_edge_y(c_r, p_r, cdx_q, cdy(gr), cdy(gr - 1), edy(gr));
+      T v1 = velocity(m1, rey);
+      const T v1_m = from_lane(v1, mc_lane);
+      if (col_ghost) v1 = v1_m;
+      const T vm1 = up(v1, 1);
+      const T vfx = momentum_flux(cdy(gr), c_fx, p_fx, up(v1, 2), vm1, v1, dn(v1, 1),
+                                  iedx_q, T(0.5) * (c_u + p_u), cdx_qm, cdx_q, dt_h);
+      const T vfx_hi = dn(vfx, 1);
+      if (own) {
+        io.mom[gr * nx + q] = advect_update(m1, dt_h, vfx_hi, vfx, edx_q, cdy(gr));
+        mput_yface(edge, io.vel, nx, ny, gr, q, v1, T(-1.0));
+      }
+      vfy_prev = vfy;
+      p_fy = c_fy;
+      p_fx = c_fx;
+      p_u = c_u;
+      p_r = c_r;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
+    }
+  }
+}
+
 template <typename K>
 int march_grid(K kernel, int& per_cu, int nitems) {
   static int cus = 0;
@@ -457,21 +888,75 @@ hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
   return hipGetLastError();
 }
 
+/* Even-tt x momentum needs faces [x0-2, x1+1] and one lane below them. */
+constexpr int kMomXW = kWave - 5;
+constexpr int kMomW = kWave - 4;
+
+template <typename T>
+hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
+                                 const double* dtp, int chunk_rows, hipStream_t s) {
+  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
+  const bool odd = (tt % 2) != 0;
+  const MarchList ml =
+      make_march(g.nx, g.ny, (!ydir && !odd) ? kMomXW : kMomW, chunk_rows);
+  if (ml.nitems < 1) return hipErrorInvalidValue;
+  if (!ydir && odd) {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momx_xy<T>, per_cu, ml.nitems);
+    hipLaunchKernelGGL(k_march_momx_xy<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+  } else if (!ydir) {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momx_yx<T>, per_cu, ml.nitems);
+    hipLaunchKernelGGL(k_march_momx_yx<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+  } else if (odd) {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momy_xy<T>, per_cu, ml.nitems);
+    hipLaunchKernelGGL(k_march_momy_xy<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+  } else {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momy_yx<T>, per_cu, ml.nitems);
+    hipLaunchKernelGGL(k_march_momy_yx<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace flowhip
 
 using namespace flowhip;
 
+static MGeo<double> mgeo64(FkMesh m) {
+  MGeo<double> g{m.nx, m.ny, m.edgedx, m.edgedy, m.celldx, m.celldy,
+                 m.inv_edgedx, m.inv_edgedy};
+  return g;
+}
+
 extern "C" {
+
+hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
+                               const double* v0, const double* F_x,
+                               const double* F_y, const double* rho,
+                               const double* mx1, double* mom_x, double* u,
+                               const double* dtp, int chunk_rows, hipStream_t s) {
+  MMomIO<double> io{u0, v0, F_x, F_y, rho, mx1, mom_x, u};
+  return launch_march_momentum<double>(mgeo64(m), io, tt, 0, dtp, chunk_rows, s);
+}
+
+hipError_t fk_march_momentum_y(FkMesh m, int tt, const double* u1,
+                               const double* v0, const double* F_x,
+                               const double* F_y, const double* rho,
+                               const double* my1, double* mom_y, double* v,
+                               const double* dtp, int chunk_rows, hipStream_t s) {
+  MMomIO<double> io{u1, v0, F_x, F_y, rho, my1, mom_y, v};
+  return launch_march_momentum<double>(mgeo64(m), io, tt, 1, dtp, chunk_rows, s);
+}
 
 hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 const double* v0, const double* rho0,
                                 const double* e1, double* rho, double* e,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s) {
-  MGeo<double> g{m.nx, m.ny, m.edgedx, m.edgedy, m.celldx, m.celldy,
-                 m.inv_edgedx, m.inv_edgedy};
   MMassIO<double> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<double>(g, io, tt, r, chunk_rows, s);
+  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, s);
 }
 
 }  // extern "C"

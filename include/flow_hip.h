@@ -24,11 +24,15 @@ extern "C" {
 #define FLOWHIP_FUSED 1     /* fused Lagrangian, advection and momentum
                                kernels; only the carried state and the
                                fluxes the momentum step reads are written */
-#define FLOWHIP_MARCH 2     /* the fused schedule with row-marching kernels */
+#define FLOWHIP_MARCH 2     /* the fused schedule with row-marching kernels
+                               (default) */
+/* The schedule of a new Mesh may also be set from the environment, for
+ * callers that only see flow_interface.h: FLOWHIP_SCHEDULE=stagewise|fused|
+ * march and FLOWHIP_MARCH_ROWS=<rows>. */
 
 int flowhip_set_schedule(Mesh* mesh, int schedule);
 int flowhip_get_schedule(Mesh* mesh);
-/* Rows per chunk of the row-marching kernels: 4..120, default 64. */
+/* Rows per chunk of the row-marching kernels: 4..120, default 32. */
 int flowhip_set_march_rows(Mesh* mesh, int rows);
 /* Replay each step from a captured hipGraph (one per sweep-order parity and
  * for the first step). */

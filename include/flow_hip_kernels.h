@@ -166,6 +166,16 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 const double* e1, double* rho, double* e,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s);
+hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
+                               const double* v0, const double* F_x,
+                               const double* F_y, const double* rho,
+                               const double* mx1, double* mom_x, double* u,
+                               const double* dtp, int chunk_rows, hipStream_t s);
+hipError_t fk_march_momentum_y(FkMesh m, int tt, const double* u1,
+                               const double* v0, const double* F_x,
+                               const double* F_y, const double* rho,
+                               const double* my1, double* mom_y, double* v,
+                               const double* dtp, int chunk_rows, hipStream_t s);
 
 #ifdef __cplusplus
 }
