@@ -21,9 +21,15 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
   double* my1 = st->Qyy;
   double* u0 = st->uF_x;
   double* v0 = st->uF_y;
-  FK_LAUNCH(h, "lagrangian",
-            fk_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1, my1, u0,
-                          v0, e1, rho0, h->red, st->max_dt, st->tt == 0, s));
+  if (h->schedule == FLOWHIP_MARCH)
+    FK_LAUNCH(h, "lagrangian",
+              fk_march_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1,
+                                  my1, u0, v0, e1, rho0, h->red, st->max_dt,
+                                  st->tt == 0, h->march_rows, s));
+  else
+    FK_LAUNCH(h, "lagrangian",
+              fk_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1, my1,
+                            u0, v0, e1, rho0, h->red, st->max_dt, st->tt == 0, s));
   if (h->schedule == FLOWHIP_MARCH)
     FK_LAUNCH(h, "advect_mass_energy",
               fk_march_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho, st->e,

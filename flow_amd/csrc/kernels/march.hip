@@ -841,6 +841,242 @@ __global__ __launch_bounds__(kMNT) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
   }
 }
 
+/* ======================================================================= */
+/* Lagrangian step                                                          */
+/* ======================================================================= */
+
+template <typename T>
+struct MLagIO {
+  const T* __restrict__ rho;
+  const T* __restrict__ e;
+  const T* __restrict__ mom_x;
+  const T* __restrict__ mom_y;
+  T* __restrict__ mx1;     /* x momentum after the viscous step */
+  T* __restrict__ my1;     /* y momentum after the viscous step */
+  T* __restrict__ u0;      /* x velocity after the viscous step, + ghosts */
+  T* __restrict__ v0;      /* y velocity after the viscous step, + ghosts */
+  T* __restrict__ e1;      /* energy after shock heating, + ghosts */
+  T* __restrict__ rho_old; /* copy of rho over every cell */
+};
+
+/* EOS, pressure acceleration, artificial viscosity, shock heating, the CFL
+ * candidates and the copy of rho (omp3/flow.c:55-70, 110-290, 73-96,
+ * 302-305). Lane l holds x face / cell column q = x0 - 3 + l (strips of 58
+ * columns: shock heating on cells [x0, x1) needs u after the viscous step on
+ * faces [x0, x1], hence Qxx on cells [x0-1, x1], u after pressure
+ * acceleration on faces [x0-2, x1+2] and the pressure one cell further
+ * west). Output row i
+ * takes, along x, u after pressure acceleration on the row's faces, Qxx on
+ * its cells and u after the viscous acceleration; along y, the lane's
+ * column carries v after pressure acceleration on y faces i .. i+3 (face
+ * i+3 computed this step), Qyy on cell rows i and i+1 and v after the
+ * viscous acceleration on faces i and i+1. Ghost faces hold the negated
+ * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
+template <typename T>
+__global__ __launch_bounds__(kMNT) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
+                                                           const double* dtp,
+                                                           FkReduce red, double max_dt,
+                                                           int first_step, MarchList ml) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  double local_min = max_dt;
+  int nan_count = 0;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves) {
+    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                      (y1 + 3 > ny - kPad);
+    const bool last_chunk = (y1 == ny - kPad);
+    const int q = x0 - 3 + lane;
+    const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
+    const int qg = min(max(q, 1), nx);
+    const bool face_ghost = is_ghost(q, sx), cell_ghost = is_ghost(q, nx);
+    const int mf_lane = mirror_lane(lane, q, sx), mc_lane = mirror_lane(lane, q, nx);
+    const bool own_face = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
+    const bool own = q >= x0 && q < x1;
+    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+
+    /* y face k of the lane's column after pressure acceleration, from cell
+     * rows k (rk, ek) and k-1 (rs, es) and mom_y at face k */
+    auto ypa = [&](int k, T rk, T ek, T rs, T es, T my, T& mom, T& rey) {
+      mom = accelerate(my, dt, eos_pressure(rk, ek), eos_pressure(rs, es), edy(k));
+      rey = rho_edge_y(rk, rs, cdx_q, cdy(k), cdy(k - 1), edy(k));
+      return velocity(mom, rey);
+    };
+
+    /* ---- prologue: y faces y0-2 .. y0+2 and v after the viscous step at y0 */
+    T wr[4], we[4];          /* rho, e at cell rows i .. i+3 */
+    T vp[5], pm[5], pr[5];   /* y faces y0-2 .. y0+2: v, mom, rho edge */
+    T qy_prev, qy_m1;        /* Qyy at cell rows i and i-1 */
+    {
+      T rr[6], ee[6];        /* cell rows y0-3 .. y0+2 */
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int row = max(y0 - 3 + s, 0);
+        rr[s] = io.rho[row * nx + qc];
+        ee[s] = io.e[row * nx + qc];
+      }
+#pragma unroll
+      for (int s = 0; s < 5; ++s) {
+        const int f = y0 - 2 + s;
+        pm[s] = pr[s] = T(0);
+        if (f >= kPad && f < ny - 1)
+          vp[s] = ypa(f, rr[s + 1], ee[s + 1], rr[s], ee[s], io.mom_y[f * nx + qc],
+                      pm[s], pr[s]);
+        else if (f == ny - 1)
+          vp[s] = -vp[s - 1 > 0 ? s - 1 : 0];
+        else if (f == ny)
+          vp[s] = -vp[s - 3 > 0 ? s - 3 : 0];
+        else
+          vp[s] = T(0);
+      }
+      if (y0 == kPad) {
+        vp[0] = -vp[3];
+        vp[1] = -vp[2];
+      }
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        wr[s] = rr[s + 3];
+        we[s] = ee[s + 3];
+      }
+      /* Qyy at cell rows y0 and y0-1 (below the south wall: row y0's) */
+      qy_prev = viscous_stress(rr[3], vp[1], vp[2], vp[3], vp[4]);
+      qy_m1 = (y0 == kPad) ? qy_prev
+                           : viscous_stress(rr[2], vp[0], vp[1], vp[2], vp[3]);
+    }
+    T vav_prev;
+    {
+      const T mom = accelerate(pm[2], dt, qy_prev, qy_m1, cdy(y0));
+      vav_prev = velocity(mom, pr[2]);
+      if (own) {
+        io.my1[y0 * nx + q] = mom;
+        mput_yface(edge, io.v0, nx, ny, y0, q, vav_prev, T(-1.0));
+      }
+    }
+    /* windows entering row y0: v at faces y0 .. y0+2; mom, rho edge at faces
+     * y0+1, y0+2 */
+    T wv[4] = {vp[2], vp[3], vp[4], T(0)};
+    T wm[3] = {pm[3], pm[4], T(0)}, wx[3] = {pr[3], pr[4], T(0)};
+    /* row y0+3 and the row-y0 x inputs */
+    {
+      const int r3 = min(y0 + 3, ny - 1);
+      wr[3] = io.rho[r3 * nx + qc];
+      we[3] = io.e[r3 * nx + qc];
+    }
+    T a_my = io.mom_y[min(y0 + 3, ny) * nx + qc];
+    T a_mx = io.mom_x[y0 * sx + qf];
+    T a_r, a_e;
+    {
+      const int r4 = min(y0 + 4, ny - 1);
+      a_r = io.rho[r4 * nx + qc];
+      a_e = io.e[r4 * nx + qc];
+    }
+
+    for (int i = y0; i < y1; ++i) {
+      const T c_my = a_my, c_mx = a_mx;
+      const T n_r = a_r, n_e = a_e;
+      {
+        a_my = io.mom_y[min(i + 4, ny) * nx + qc];
+        a_mx = io.mom_x[min(i + 1, ny - 1) * sx + qf];
+        const int r5 = min(i + 5, ny - 1);
+        a_r = io.rho[r5 * nx + qc];
+        a_e = io.e[r5 * nx + qc];
+      }
+      /* y face i+3 after pressure acceleration */
+      const int k = i + 3;
+      T vk;
+      if (k < ny - 1) {
+        vk = ypa(k, wr[3], we[3], wr[2], we[2], c_my, wm[2], wx[2]);
+      } else {
+        vk = -((k == ny - 1) ? wv[2] : wv[0]);
+        wm[2] = wx[2] = T(0);
+      }
+      wv[3] = vk;
+      /* Qyy at cell row i+1 (a ghost row is its mirror, row i) */
+      const T qy = (i + 1 >= ny - kPad) ? qy_prev
+                                         : viscous_stress(wr[1], wv[0], wv[1], wv[2], wv[3]);
+      /* y face i+1 after the viscous acceleration */
+      const T my_av = accelerate(wm[0], dt, qy, qy_prev, cdy(i + 1));
+      const T vav = velocity(my_av, wx[0]);
+      if (own && (i + 1 < y1 || last_chunk)) {
+        io.my1[(i + 1) * nx + q] = my_av;
+        mput_yface(edge, io.v0, nx, ny, i + 1, q, vav, T(-1.0));
+      }
+
+      /* along x on row i */
+      const T ri = wr[0], ei = we[0];
+      const T P = eos_pressure(ri, ei);
+      const T mx_pa = accelerate(c_mx, dt, P, up(P, 1), edx_q);
+      const T rex = rho_edge_x(ri, up(ri, 1), cdx_q, cdx_qm, cdy(i), edx_q);
+      T u_pa = velocity(mx_pa, rex);
+      const T u_pa_m = from_lane(u_pa, mf_lane);
+      if (face_ghost) u_pa = -u_pa_m;
+      T qx = viscous_stress(ri, up(u_pa, 1), u_pa, dn(u_pa, 1), dn(u_pa, 2));
+      const T qx_m = from_lane(qx, mc_lane);
+      if (cell_ghost) qx = qx_m;
+      const T mx_av = accelerate(mx_pa, dt, qx, up(qx, 1), cdx_q);
+      const T uav = velocity(mx_av, rex);
+      if (own_face) {
+        io.mx1[i * sx + q] = mx_av;
+        mput_xface(edge, io.u0, nx, ny, i, q, uav, T(-1.0));
+      }
+
+      /* shock heating, CFL candidate and the copy of rho on row i */
+      const T uav_e = dn(uav, 1);
+      if (own) {
+        const int c = i * nx + q;
+        const T es = shock_heating(ei, P, ri, qx, qy_prev, uav, uav_e, vav_prev, vav,
+                                   cdx_q, cdy(i), dt_h);
+        mput_cell(edge, io.e1, nx, ny, i, q, es);
+        io.rho_old[c] = ri;
+        if (edge) {
+          const bool w = (q == kPad), ea = (q == nx - kPad - 1);
+          const bool s = (i == kPad), no = (i == ny - kPad - 1);
+          for (int d = 1; d <= kPad; ++d) {
+            if (w) io.rho_old[c - d] = io.rho[c - d];
+            if (ea) io.rho_old[c + d] = io.rho[c + d];
+            if (s) io.rho_old[c - d * nx] = io.rho[c - d * nx];
+            if (no) io.rho_old[c + d * nx] = io.rho[c + d * nx];
+            for (int d2 = 1; d2 <= kPad; ++d2) {
+              if (s && w) io.rho_old[c - d * nx - d2] = io.rho[c - d * nx - d2];
+              if (s && ea) io.rho_old[c - d * nx + d2] = io.rho[c - d * nx + d2];
+              if (no && w) io.rho_old[c + d * nx - d2] = io.rho[c + d * nx - d2];
+              if (no && ea) io.rho_old[c + d * nx + d2] = io.rho[c + d * nx + d2];
+            }
+          }
+        }
+        const double cand = cell_timestep(double(es), double(qx), double(qy_prev),
+                                          double(ri), double(cdx_q), double(cdy(i)));
+        if (cand == cand)
+          local_min = lesser(local_min, cand);
+        else
+          ++nan_count;
+      }
+
+      /* advance the windows */
+      qy_prev = qy;
+      vav_prev = vav;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        wv[s] = wv[s + 1];
+        wr[s] = wr[s + 1];
+        we[s] = we[s + 1];
+      }
+      wr[3] = n_r;
+      we[3] = n_e;
+      wm[0] = wm[1];
+      wm[1] = wm[2];
+      wx[0] = wx[1];
+      wx[1] = wx[2];
+    }
+  }
+  finish_timestep<kMNT>(local_min, nan_count, red, max_dt, first_step);
+}
+
 template <typename K>
 int march_grid(K kernel, int& per_cu, int nitems) {
   static int cus = 0;
@@ -888,6 +1124,19 @@ hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
+                                   int first_step, int chunk_rows, hipStream_t s) {
+  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
+  const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
+  if (ml.nitems < 1) return hipErrorInvalidValue;
+  static int per_cu = 0;
+  const int grid = march_grid(k_march_lagrangian<T>, per_cu, ml.nitems);
+  hipLaunchKernelGGL(k_march_lagrangian<T>, dim3(grid), dim3(kMNT), 0, s, g, io,
+                     (const double*)r.dtp, r, max_dt, first_step, ml);
+  return hipGetLastError();
+}
+
 /* Even-tt x momentum needs faces [x0-2, x1+1] and one lane below them. */
 constexpr int kMomXW = kWave - 5;
 constexpr int kMomW = kWave - 4;
@@ -931,6 +1180,16 @@ static MGeo<double> mgeo64(FkMesh m) {
 }
 
 extern "C" {
+
+hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
+                               const double* mom_x, const double* mom_y, double* mx1,
+                               double* my1, double* u0, double* v0, double* e1,
+                               double* rho_old, FkReduce r, double max_dt,
+                               int first_step, int chunk_rows, hipStream_t s) {
+  MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
+  return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
+                                         chunk_rows, s);
+}
 
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
                                const double* v0, const double* F_x,

@@ -166,6 +166,11 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 const double* e1, double* rho, double* e,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s);
+hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
+                               const double* mom_x, const double* mom_y, double* mx1,
+                               double* my1, double* u0, double* v0, double* e1,
+                               double* rho_old, FkReduce r, double max_dt,
+                               int first_step, int chunk_rows, hipStream_t s);
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
                                const double* v0, const double* F_x,
                                const double* F_y, const double* rho,
