@@ -42,8 +42,9 @@ KERNEL_SWEEPS = {
     "mom_flux_xx": 3, "mom_flux_xy": 4, "mom_flux_yx": 4, "mom_flux_yy": 3,
     "mom_x_update_u": 5, "mom_y_update_v": 5, "mom_x_update": 3,
     "mom_y_update": 3, "conservation": 2,
-    # fused schedule
-    "lagrangian": 9, "advect_mass_energy": 8, "advect_momentum_x": 8,
+    # fused and march schedules: the Lagrangian kernel reads rho, e, mom_x,
+    # mom_y and writes mx1, my1, u0, v0, e1, rho_old
+    "lagrangian": 10, "advect_mass_energy": 8, "advect_momentum_x": 8,
     "advect_momentum_y": 8, "advect_momentum": 11,
 }
 
@@ -79,7 +80,8 @@ def reset_state(st, rho0, e0, mesh, dt_param):
 
 def traffic_from_profiles(kernel, n):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE (scaled by a copy-kernel calibration), if committed."""
+    WRITE_SIZE (scaled by a copy-kernel calibration), if committed; kernel
+    is "<schedule>:<timer name>"."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
     if not os.path.exists(path):
         return None
@@ -175,7 +177,7 @@ def main():
     achieved = alg_bytes / avg_s
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": traffic_from_profiles(dom, n),
+                "traffic": traffic_from_profiles(f"{sched_name}:{dom}", n),
                 "kernel": dom, "avg_launch_ms": dom_ms / dom_launches,
                 "alg_bytes_per_launch": alg_bytes}
     kernels = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / args.timing_steps}

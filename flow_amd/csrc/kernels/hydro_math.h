@@ -39,6 +39,18 @@ __device__ __forceinline__ T eos_pressure(T rho, T e) {
   return T(kGam - 1.0) * rho * e;
 }
 
+/* Density below which the FP32-state form evaluates the EOS in FP64: the
+ * product of a near-vacuum density and its energy loses its low bits (or
+ * underflows) in FP32 before it is rounded back. */
+constexpr float kNearVacuumRho = 1.0e-6f;
+
+template <>
+__device__ __forceinline__ float eos_pressure<float>(float rho, float e) {
+  if (rho < kNearVacuumRho)
+    return float((kGam - 1.0) * double(rho) * double(e));
+  return float(kGam - 1.0) * rho * e;
+}
+
 /* Edge-centred density on an x face between cells j-1 (w) and j (c):
  * omp3/flow.c:134-137. */
 template <typename T>

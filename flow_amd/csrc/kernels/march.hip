@@ -1218,4 +1218,50 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
   return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, s);
 }
 
+static MGeo<float> mgeo32(FkMesh32 m) {
+  MGeo<float> g{m.nx, m.ny, m.edgedx, m.edgedy, m.celldx, m.celldy,
+                m.inv_edgedx, m.inv_edgedy};
+  return g;
+}
+
+hipError_t fk_march_lagrangian_f32(FkMesh32 m, const float* rho, const float* e,
+                                   const float* mom_x, const float* mom_y,
+                                   float* mx1, float* my1, float* u0, float* v0,
+                                   float* e1, float* rho_old, FkReduce r,
+                                   double max_dt, int first_step, int chunk_rows,
+                                   hipStream_t s) {
+  MLagIO<float> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
+  return launch_march_lagrangian<float>(mgeo32(m), io, r, max_dt, first_step,
+                                        chunk_rows, s);
+}
+
+hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
+                                    const float* v0, const float* rho0,
+                                    const float* e1, float* rho, float* e,
+                                    float* F_x, float* F_y, FkReduce r,
+                                    int chunk_rows, hipStream_t s) {
+  MMassIO<float> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
+  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, s);
+}
+
+hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
+                                   const float* v0, const float* F_x,
+                                   const float* F_y, const float* rho,
+                                   const float* mx1, float* mom_x, float* u,
+                                   const double* dtp, int chunk_rows,
+                                   hipStream_t s) {
+  MMomIO<float> io{u0, v0, F_x, F_y, rho, mx1, mom_x, u};
+  return launch_march_momentum<float>(mgeo32(m), io, tt, 0, dtp, chunk_rows, s);
+}
+
+hipError_t fk_march_momentum_y_f32(FkMesh32 m, int tt, const float* u1,
+                                   const float* v0, const float* F_x,
+                                   const float* F_y, const float* rho,
+                                   const float* my1, float* mom_y, float* v,
+                                   const double* dtp, int chunk_rows,
+                                   hipStream_t s) {
+  MMomIO<float> io{u1, v0, F_x, F_y, rho, my1, mom_y, v};
+  return launch_march_momentum<float>(mgeo32(m), io, tt, 1, dtp, chunk_rows, s);
+}
+
 }  // extern "C"

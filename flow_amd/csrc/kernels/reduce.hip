@@ -21,9 +21,10 @@ constexpr int kRedBlocks = 1024;
  * candidates (cells with negative energy: sqrt of a negative) are left out,
  * as the reference's min(a, b) = a < b ? a : b leaves them out in effect,
  * and counted. */
+template <typename M, typename T>
 __global__ __launch_bounds__(kRedThreads) void k_timestep(
-    FkMesh m, const double* __restrict__ Qxx, const double* __restrict__ Qyy,
-    const double* __restrict__ rho, const double* __restrict__ e, FkReduce r,
+    M m, const T* __restrict__ Qxx, const T* __restrict__ Qyy,
+    const T* __restrict__ rho, const T* __restrict__ e, FkReduce r,
     double max_dt, int first_step) {
   const int pad = m.pad, wx = m.nx - 2 * pad, wy = m.ny - 2 * pad;
   const long n = (long)wx * wy;
@@ -33,8 +34,9 @@ __global__ __launch_bounds__(kRedThreads) void k_timestep(
        k += (long)gridDim.x * blockDim.x) {
     const int i = pad + (int)(k / wx), j = pad + (int)(k % wx);
     const int c = i * m.nx + j;
-    const double cand =
-        cell_timestep(e[c], Qxx[c], Qyy[c], rho[c], m.celldx[j], m.celldy[i]);
+    const double cand = cell_timestep(double(e[c]), double(Qxx[c]), double(Qyy[c]),
+                                      double(rho[c]), double(m.celldx[j]),
+                                      double(m.celldy[i]));
     if (cand == cand)
       local = lesser(local, cand);
     else
@@ -43,9 +45,9 @@ __global__ __launch_bounds__(kRedThreads) void k_timestep(
   finish_timestep<kRedThreads>(local, bad, r, max_dt, first_step);
 }
 
+template <typename M, typename T>
 __global__ __launch_bounds__(kRedThreads) void k_conservation(
-    FkMesh m, const double* __restrict__ rho, const double* __restrict__ e,
-    FkReduce r) {
+    M m, const T* __restrict__ rho, const T* __restrict__ e, FkReduce r) {
   const int pad = m.pad, wx = m.nx - 2 * pad, wy = m.ny - 2 * pad;
   const long n = (long)wx * wy;
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
@@ -53,8 +55,8 @@ __global__ __launch_bounds__(kRedThreads) void k_conservation(
        k += (long)gridDim.x * blockDim.x) {
     const int i = pad + (int)(k / wx), j = pad + (int)(k % wx);
     const int c = i * m.nx + j;
-    ms.add(rho[c]);
-    es.add(rho[c] * e[c]);
+    ms.add(double(rho[c]));
+    es.add(double(rho[c]) * double(e[c]));
   }
   finish_sums<kRedThreads>(ms, es, r);
 }
@@ -78,16 +80,35 @@ hipError_t fk_timestep(FkMesh m, const double* Qxx, const double* Qyy,
                        const double* rho, const double* e, FkReduce r,
                        double max_dt, int first_step, hipStream_t s) {
   const long n = (long)(m.nx - 2 * m.pad) * (m.ny - 2 * m.pad);
-  hipLaunchKernelGGL(k_timestep, dim3(red_blocks(n)), dim3(kRedThreads), 0, s,
-                     m, Qxx, Qyy, rho, e, r, max_dt, first_step);
+  hipLaunchKernelGGL((k_timestep<FkMesh, double>), dim3(red_blocks(n)),
+                     dim3(kRedThreads), 0, s, m, Qxx, Qyy, rho, e, r, max_dt,
+                     first_step);
   return hipGetLastError();
 }
 
 hipError_t fk_conservation(FkMesh m, const double* rho, const double* e,
                            FkReduce r, hipStream_t s) {
   const long n = (long)(m.nx - 2 * m.pad) * (m.ny - 2 * m.pad);
-  hipLaunchKernelGGL(k_conservation, dim3(red_blocks(n)), dim3(kRedThreads), 0,
-                     s, m, rho, e, r);
+  hipLaunchKernelGGL((k_conservation<FkMesh, double>), dim3(red_blocks(n)),
+                     dim3(kRedThreads), 0, s, m, rho, e, r);
+  return hipGetLastError();
+}
+
+hipError_t fk_timestep_f32(FkMesh32 m, const float* Qxx, const float* Qyy,
+                           const float* rho, const float* e, FkReduce r,
+                           double max_dt, int first_step, hipStream_t s) {
+  const long n = (long)(m.nx - 2 * m.pad) * (m.ny - 2 * m.pad);
+  hipLaunchKernelGGL((k_timestep<FkMesh32, float>), dim3(red_blocks(n)),
+                     dim3(kRedThreads), 0, s, m, Qxx, Qyy, rho, e, r, max_dt,
+                     first_step);
+  return hipGetLastError();
+}
+
+hipError_t fk_conservation_f32(FkMesh32 m, const float* rho, const float* e,
+                               FkReduce r, hipStream_t s) {
+  const long n = (long)(m.nx - 2 * m.pad) * (m.ny - 2 * m.pad);
+  hipLaunchKernelGGL((k_conservation<FkMesh32, float>), dim3(red_blocks(n)),
+                     dim3(kRedThreads), 0, s, m, rho, e, r);
   return hipGetLastError();
 }
 

@@ -21,8 +21,9 @@ namespace {
 /* arch handle_boundary_2d on one rank: pad-deep mirror strips; north/south
  * over columns [pad, nxp-pad), east/west over rows [pad, nyp-pad); corners
  * untouched. One thread per ghost element, all four strips in one grid. */
-__global__ void k_boundary(double* __restrict__ a, int nxp, int nyp, int pad,
-                           double sx, double sy) {
+template <typename T>
+__global__ void k_boundary(T* __restrict__ a, int nxp, int nyp, int pad, T sx,
+                           T sy) {
   const int wx = nxp - 2 * pad, wy = nyp - 2 * pad;
   const int n_ns = pad * wx, n_ew = pad * wy;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -54,10 +55,12 @@ __global__ void k_reciprocal(int n, double* __restrict__ dst,
   if (k < n) dst[k] = 1.0 / src[k];
 }
 
-__global__ void k_copy(long n, double* __restrict__ dst,
-                       const double* __restrict__ src) {
+/* dst = src converted to the destination type (round to nearest when
+ * narrowing); a plain copy when the types agree. */
+template <typename D, typename S>
+__global__ void k_copy(long n, D* __restrict__ dst, const S* __restrict__ src) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) dst[k] = src[k];
+  if (k < n) dst[k] = D(src[k]);
 }
 
 __global__ void k_eos(Domain d, int nx, double* __restrict__ P,
@@ -335,8 +338,33 @@ hipError_t fk_boundary(int nxp, int nyp, int pad, double* arr, int invert,
   if (n <= 0) return hipSuccess;
   const double sx = (invert == 1) ? -1.0 : 1.0;
   const double sy = (invert == 2) ? -1.0 : 1.0;
-  hipLaunchKernelGGL(k_boundary, dim3((n + 255) / 256), dim3(256), 0, s, arr,
-                     nxp, nyp, pad, sx, sy);
+  hipLaunchKernelGGL(k_boundary<double>, dim3((n + 255) / 256), dim3(256), 0, s,
+                     arr, nxp, nyp, pad, sx, sy);
+  return hipGetLastError();
+}
+
+hipError_t fk_boundary_f32(int nxp, int nyp, int pad, float* arr, int invert,
+                           hipStream_t s) {
+  const int n = 2 * pad * (nxp - 2 * pad) + 2 * pad * (nyp - 2 * pad);
+  if (n <= 0) return hipSuccess;
+  const float sx = (invert == 1) ? -1.0f : 1.0f;
+  const float sy = (invert == 2) ? -1.0f : 1.0f;
+  hipLaunchKernelGGL(k_boundary<float>, dim3((n + 255) / 256), dim3(256), 0, s,
+                     arr, nxp, nyp, pad, sx, sy);
+  return hipGetLastError();
+}
+
+hipError_t fk_narrow(long n, float* dst, const double* src, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_copy<float, double>), dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, n, dst, src);
+  return hipGetLastError();
+}
+
+hipError_t fk_widen(long n, double* dst, const float* src, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_copy<double, float>), dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, n, dst, src);
   return hipGetLastError();
 }
 
@@ -350,8 +378,8 @@ hipError_t fk_reciprocal(int n, double* dst, const double* src,
 
 hipError_t fk_copy(long n, double* dst, const double* src, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     s, n, dst, src);
+  hipLaunchKernelGGL((k_copy<double, double>), dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, n, dst, src);
   return hipGetLastError();
 }
 

@@ -29,6 +29,14 @@ typedef struct FkMesh {
   const double *inv_edgedx, *inv_edgedy;
 } FkMesh;
 
+/* Geometry of the FP32-state precision form: the same arrays rounded to
+ * FP32 (the reciprocals are rounded from the FP64 reciprocals). */
+typedef struct FkMesh32 {
+  int nx, ny, pad;
+  const float *edgedx, *edgedy, *celldx, *celldy;
+  const float *inv_edgedx, *inv_edgedy;
+} FkMesh32;
+
 /* Device scratch for the single-pass grid reductions. */
 typedef struct FkReduce {
   double* partials;      /* fk_reduce_blocks() doubles */
@@ -181,6 +189,45 @@ hipError_t fk_march_momentum_y(FkMesh m, int tt, const double* u1,
                                const double* F_y, const double* rho,
                                const double* my1, double* mom_y, double* v,
                                const double* dtp, int chunk_rows, hipStream_t s);
+
+/* ---- FP32-state precision form (march kernels over float arrays) ----
+ * Same contracts as the FP64 launchers. The CFL candidates, the dt
+ * reduction and the conservation totals are computed in FP64 from the FP32
+ * state, the EOS in FP64 at near-vacuum cells (rho < 1e-6). */
+hipError_t fk_timestep_f32(FkMesh32 m, const float* Qxx, const float* Qyy,
+                           const float* rho, const float* e, FkReduce r,
+                           double max_dt, int first_step, hipStream_t s);
+hipError_t fk_conservation_f32(FkMesh32 m, const float* rho, const float* e,
+                               FkReduce r, hipStream_t s);
+hipError_t fk_march_lagrangian_f32(FkMesh32 m, const float* rho, const float* e,
+                                   const float* mom_x, const float* mom_y,
+                                   float* mx1, float* my1, float* u0, float* v0,
+                                   float* e1, float* rho_old, FkReduce r,
+                                   double max_dt, int first_step, int chunk_rows,
+                                   hipStream_t s);
+hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
+                                    const float* v0, const float* rho0,
+                                    const float* e1, float* rho, float* e,
+                                    float* F_x, float* F_y, FkReduce r,
+                                    int chunk_rows, hipStream_t s);
+hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
+                                   const float* v0, const float* F_x,
+                                   const float* F_y, const float* rho,
+                                   const float* mx1, float* mom_x, float* u,
+                                   const double* dtp, int chunk_rows,
+                                   hipStream_t s);
+hipError_t fk_march_momentum_y_f32(FkMesh32 m, int tt, const float* u1,
+                                   const float* v0, const float* F_x,
+                                   const float* F_y, const float* rho,
+                                   const float* my1, float* mom_y, float* v,
+                                   const double* dtp, int chunk_rows,
+                                   hipStream_t s);
+/* dst[k] = (float)src[k] / (double)src[k], k < n (round to nearest). */
+hipError_t fk_narrow(long n, float* dst, const double* src, hipStream_t s);
+hipError_t fk_widen(long n, double* dst, const float* src, hipStream_t s);
+/* arch handle_boundary_2d on an FP32 array. */
+hipError_t fk_boundary_f32(int nxp, int nyp, int pad, float* arr, int invert,
+                           hipStream_t s);
 
 #ifdef __cplusplus
 }
