@@ -65,6 +65,8 @@ def parse_args():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="approximate CPU time budget of the oracle sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-fp32", action="store_true",
+                   help="skip the FP32-state precision form")
     return p.parse_args()
 
 
@@ -91,6 +93,82 @@ def traffic_from_profiles(kernel, n):
         if m and m.group(1) == kernel and int(m.group(2)) == n:
             return float(m.group(3))
     return None
+
+
+def field_errors(got, ref, L):
+    """Per-field relative L-inf and L1 error of got against ref over the
+    regions omp3 defines (rho, e, u, v)."""
+    from tests import harness
+    g, r = harness.views(got, L), harness.views(ref, L)
+    reg = harness.state_regions(L)
+    out = {}
+    for k in ("rho", "e", "u", "v"):
+        a, b = g[k][reg[k]], r[k][reg[k]]
+        d = np.abs(a - b)
+        out[k] = {"rel_linf": float(d.max() / np.abs(b).max()),
+                  "rel_l1": float(d.sum() / np.abs(b).sum())}
+    return out
+
+
+def fp32_leg(args, n, rho0, e0, ref_state, ref_dts, ref_mass, ref_energy,
+             fp64_cups):
+    """The FP32-state precision form on the same workload: timed like the
+    FP64 run, then its state after the same steps against the FP64 state."""
+    import flow_amd
+    from flow_amd import native
+    mesh = flow_amd.Mesh(n, n, dt=args.dt)
+    if args.march_rows:
+        mesh.set_march_rows(args.march_rows)
+    st = flow_amd.DeviceState32(mesh)
+
+    def reset():
+        for k in flow_amd.ARRAY_NAMES:
+            st.zero(k)
+        st.upload("rho", rho0)
+        st.upload("e", e0)
+        st.reflect_initial()
+        mesh.set_dt(0.3 * args.dt, 0.3 * args.dt)
+
+    reset()
+    if args.warmup:
+        st.run(args.warmup)
+    reset()
+    st.run(0)  # the pre-loop set_timestep
+    native.sync()
+    t0 = time.perf_counter()
+    steps, dts, mass, energy, _ = st.run(args.steps, pre_timestep=False)
+    native.sync()
+    elapsed = time.perf_counter() - t0
+    if steps != args.steps:
+        raise RuntimeError(f"FP32 run stopped after {steps} steps")
+    cups = n * n * steps / elapsed
+    got = {k: st.download(k) for k in ("rho", "e", "u", "v", "mom_x", "mom_y")}
+    errs = field_errors(got, ref_state, n + 4)
+    mesh.reset_timing()
+    mesh.set_timing(True)
+    reset()
+    st.run(args.timing_steps)
+    mesh.set_timing(False)
+    timers = mesh.timing()
+    st.close()
+    mesh.close()
+    return {
+        "value": cups, "unit": "cell-updates/s", "ms_per_step": elapsed / steps * 1e3,
+        "speedup_vs_fp64": cups / fp64_cups,
+        "step_roofline": {"b_alg_per_cell_update": B_ALG_FP32,
+                          "achieved_GBps": cups * B_ALG_FP32 / 1e9,
+                          "frac": cups * B_ALG_FP32 / HBM_PEAK},
+        "error_vs_fp64": errs,
+        "max_field_rel_linf": max(v["rel_linf"] for v in errs.values()),
+        "mass_rel_diff": float(abs(mass[-1] - ref_mass[-1]) / abs(ref_mass[-1])),
+        "energy_rel_diff": float(abs(energy[-1] - ref_energy[-1]) / abs(ref_energy[-1])),
+        "mass_drift": float((mass[-1] - mass[0]) / mass[0]),
+        "energy_drift": float((energy[-1] - energy[0]) / energy[0]),
+        "dt_rel_diff_max": float(np.max(np.abs(dts[:, 0] - ref_dts[:, 0]) / ref_dts[:, 0])),
+        "kernels": {k: v[0] / v[1] for k, v in timers.items()},
+        "comparator": "the FP64 GPU state after the same steps (bitwise equal "
+                      "to the oracle: parity_vs_oracle)",
+    }
 
 
 def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
@@ -160,6 +238,11 @@ def main():
         raise RuntimeError(f"run stopped after {steps} steps")
     elapsed = t1 - t0
     cups = n * n * steps / elapsed
+    fp32 = None
+    if not args.no_fp32:
+        ref_state = {k: st.download(k) for k in ("rho", "e", "u", "v", "mom_x", "mom_y")}
+        fp32 = fp32_leg(args, n, rho0, e0, ref_state, dts, mass, energy, cups)
+        del ref_state
 
     # per-kernel HIP-event timing over the same workload
     reset_state(st, rho0, e0, mesh, args.dt)
@@ -199,6 +282,7 @@ def main():
                           "achieved_GBps": cups * B_ALG_FP64 / 1e9,
                           "frac": cups * B_ALG_FP64 / HBM_PEAK},
         "kernels": kernels,
+        "fp32_state": fp32,
         "device": native.device_name(),
     }
 

@@ -7,7 +7,8 @@ computes the timestep itself, and importing the runtime fails loudly when the
 library has not been built.
 """
 from .native import (  # noqa: F401
-    ARRAY_NAMES, FUSED, MARCH, STAGEWISE, Mesh, MeshStruct, DeviceState, lib,
+    ARRAY_NAMES, FUSED, MARCH, STAGEWISE, Mesh, MeshStruct, DeviceState,
+    DeviceState32, lib,
     library_path,
 )
 from . import problems  # noqa: F401

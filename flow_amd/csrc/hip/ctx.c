@@ -78,6 +78,7 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   }
   hipFree(h->inv_edgedx);
   hipFree(h->inv_edgedy);
+  hipFree(h->geo32);
   hipFree(h->red.partials);
   hipFree(h->red.ticket);
   hipFree(h->red.dtp);
@@ -218,4 +219,42 @@ int flowhip_timing_reset(Mesh* mesh) {
   }
   h->ntimers = 0;
   return 0;
+}
+
+/* The geometry rounded to FP32, rebuilt when the mesh's arrays change. */
+FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
+  FlowHipCtx* h = flowhip_ctx(mesh);
+  const FkMesh m = flowhip_fkmesh(mesh);
+  const int lx = mesh->local_nx + 1, ly = mesh->local_ny + 1;
+  if (!(h->geo32 && h->geo32_src[0] == mesh->edgedx &&
+        h->geo32_src[1] == mesh->edgedy && h->geo32_src[2] == mesh->celldx &&
+        h->geo32_src[3] == mesh->celldy && h->geo32_nx == lx &&
+        h->geo32_ny == ly)) {
+    hipFree(h->geo32);
+    HIPCHECK(hipMalloc((void**)&h->geo32, sizeof(float) * 3 * (lx + ly)));
+    float* g = h->geo32;
+    HIPCHECK(fk_narrow(lx, g, m.edgedx, h->stream));
+    HIPCHECK(fk_narrow(ly, g + lx, m.edgedy, h->stream));
+    HIPCHECK(fk_narrow(lx, g + lx + ly, m.celldx, h->stream));
+    HIPCHECK(fk_narrow(ly, g + 2 * lx + ly, m.celldy, h->stream));
+    HIPCHECK(fk_narrow(lx, g + 2 * (lx + ly), m.inv_edgedx, h->stream));
+    HIPCHECK(fk_narrow(ly, g + 3 * lx + 2 * ly, m.inv_edgedy, h->stream));
+    h->geo32_src[0] = mesh->edgedx;
+    h->geo32_src[1] = mesh->edgedy;
+    h->geo32_src[2] = mesh->celldx;
+    h->geo32_src[3] = mesh->celldy;
+    h->geo32_nx = lx;
+    h->geo32_ny = ly;
+  }
+  FkMesh32 f;
+  f.nx = m.nx;
+  f.ny = m.ny;
+  f.pad = m.pad;
+  f.edgedx = h->geo32;
+  f.edgedy = h->geo32 + lx;
+  f.celldx = h->geo32 + lx + ly;
+  f.celldy = h->geo32 + 2 * lx + ly;
+  f.inv_edgedx = h->geo32 + 2 * (lx + ly);
+  f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
+  return f;
 }

@@ -37,6 +37,11 @@ typedef struct FlowHipCtx {
   double *inv_edgedx, *inv_edgedy; /* 1/edgedx, 1/edgedy of the mesh */
   const double *inv_src_x, *inv_src_y;
   int inv_nx, inv_ny;
+  /* FP32 copies of the geometry for the precision form: edgedx, edgedy,
+   * celldx, celldy, 1/edgedx, 1/edgedy, each (L+1) floats */
+  float* geo32;
+  const double* geo32_src[4];
+  int geo32_nx, geo32_ny;
   double* h_stage;      /* pinned: [0..1] dt pair, [2..5] sums */
   int* h_status;        /* pinned */
   int schedule;
@@ -62,6 +67,7 @@ typedef struct FlowHipCtx {
 FlowHipCtx* flowhip_ctx(Mesh* mesh);
 void flowhip_ctx_destroy(Mesh* mesh);
 FkMesh flowhip_fkmesh(const Mesh* mesh);
+FkMesh32 flowhip_fkmesh32(Mesh* mesh);
 
 /* Timing brackets around one launch; no-ops unless timing is on. */
 int flowhip_timer_begin(FlowHipCtx* h, const char* name);

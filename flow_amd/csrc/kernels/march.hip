@@ -61,13 +61,39 @@ inline MarchList make_march(int nx, int ny, int W, int H) {
   return m;
 }
 
+/* Whole-wave shifts by one lane as DPP moves (wave_shr:1 / wave_shl:1):
+ * VALU operations on registers, with none of the LDS round trip of a
+ * ds_bpermute. Lane 0 of a shift up and lane 63 of a shift down keep their
+ * own value; no kernel reads those lanes' results. */
+constexpr int kDppWaveShl1 = 0x130; /* lane i <- lane i+1 */
+constexpr int kDppWaveShr1 = 0x138; /* lane i <- lane i-1 */
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL>((int)b);
+  const int hi = dpp_move<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+
+/* value of the lane d below (up) or above (dn), d = 1 or 2 */
 template <typename T>
 __device__ __forceinline__ T up(T v, int d) {
-  return __shfl_up(v, d, kWave);
+  const T v1 = dpp_move<kDppWaveShr1>(v);
+  return d == 1 ? v1 : dpp_move<kDppWaveShr1>(v1);
 }
 template <typename T>
 __device__ __forceinline__ T dn(T v, int d) {
-  return __shfl_down(v, d, kWave);
+  const T v1 = dpp_move<kDppWaveShl1>(v);
+  return d == 1 ? v1 : dpp_move<kDppWaveShl1>(v1);
 }
 
 
@@ -433,6 +459,16 @@ __device__ __forceinline__ T from_lane(T v, int src) {
   return __shfl(v, src, kWave);
 }
 
+/* A ghost lane's value: sign times its mirror lane's. Only strips within
+ * three columns of a wall hold ghost lanes (xedge, uniform over the wave),
+ * so only they pay for the cross-lane permute. */
+template <typename T>
+__device__ __forceinline__ T mirror_fix(bool xedge, T v, int src, bool ghost, T sign) {
+  if (!xedge) return v;
+  const T m = from_lane(v, src);
+  return ghost ? sign * m : v;
+}
+
 /* lane holding the mirror of index q (lane holds q), kept inside the wave */
 __device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
   return min(max(lane + (mirror(q, n) - q), 0), kWave - 1);
@@ -458,6 +494,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
     const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
     const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                       (y1 + 3 > ny - kPad);
+    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
     const int q = x0 - 2 + lane;
     const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
     const bool cell_ghost = is_ghost(q, nx);
@@ -474,8 +511,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
       T ufx = momentum_flux(edy(i), a_fx, dn(a_fx, 1), up(a_u, 1), a_u, up1,
                             dn(a_u, 2), iedx_q, T(0.5) * (a_u + up1), cdx_qm,
                             cdx_q, dt_h);
-      const T ufx_m = from_lane(ufx, mc_lane);
-      if (cell_ghost) ufx = ufx_m;
+      ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
       m1 = advect_update(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i));
       const T rex = rho_edge_x(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q);
       u1 = velocity(m1, rex);
@@ -554,6 +590,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
     const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
     const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                       (y1 + 3 > ny - kPad);
+    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
     const int q = x0 - 3 + lane;
     const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
     const int qg = min(max(q, 1), nx);
@@ -599,13 +636,11 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
       const T m1 = advect_update(c_m1, dt_h, uf, uf_prev, cdx_q, edy(r));
       const T rex = rho_edge_x(c_r, up(c_r, 1), cdx_q, cdx_qm, cdy(r), edx_q);
       T u1 = velocity(m1, rex);
-      const T u1_m = from_lane(u1, mf_lane);
-      if (face_ghost) u1 = -u1_m;
+      u1 = mirror_fix(xedge, u1, mf_lane, face_ghost, T(-1.0));
       const T up1 = dn(u1, 1);
       T ufx = momentum_flux(edy(r), c_fx, dn(c_fx, 1), up(u1, 1), u1, up1, dn(u1, 2),
                             iedx_q, T(0.5) * (u1 + up1), cdx_qm, cdx_q, dt_h);
-      const T ufx_m = from_lane(ufx, mc_lane);
-      if (cell_ghost) ufx = ufx_m;
+      ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
       const T ufx_lo = up(ufx, 1);
       if (own) {
         io.mom[r * sx + q] = advect_update(m1, dt_h, ufx, ufx_lo, edx_q, cdy(r));
@@ -769,6 +804,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
     const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
     const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                       (y1 + 3 > ny - kPad);
+    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
     const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
     const int q = x0 - 2 + lane;
     const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
@@ -820,8 +856,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
       const T m1 = advect_update(c_m1, dt_h, vfy, vfy_prev, cdx_q, edy(gr));
       const T rey = rho_edge_y(c_r, p_r, cdx_q, cdy(gr), cdy(gr - 1), edy(gr));
       T v1 = velocity(m1, rey);
-      const T v1_m = from_lane(v1, mc_lane);
-      if (col_ghost) v1 = v1_m;
+      v1 = mirror_fix(xedge, v1, mc_lane, col_ghost, T(1.0));
       const T vm1 = up(v1, 1);
       const T vfx = momentum_flux(cdy(gr), c_fx, p_fx, up(v1, 2), vm1, v1, dn(v1, 1),
                                   iedx_q, T(0.5) * (c_u + p_u), cdx_qm, cdx_q, dt_h);
@@ -889,6 +924,7 @@ __global__ __launch_bounds__(kMNT) void k_march_lagrangian(MGeo<T> g, MLagIO<T> 
     const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
     const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                       (y1 + 3 > ny - kPad);
+    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
     const bool last_chunk = (y1 == ny - kPad);
     const int q = x0 - 3 + lane;
     const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
@@ -1013,11 +1049,9 @@ __global__ __launch_bounds__(kMNT) void k_march_lagrangian(MGeo<T> g, MLagIO<T> 
       const T mx_pa = accelerate(c_mx, dt, P, up(P, 1), edx_q);
       const T rex = rho_edge_x(ri, up(ri, 1), cdx_q, cdx_qm, cdy(i), edx_q);
       T u_pa = velocity(mx_pa, rex);
-      const T u_pa_m = from_lane(u_pa, mf_lane);
-      if (face_ghost) u_pa = -u_pa_m;
+      u_pa = mirror_fix(xedge, u_pa, mf_lane, face_ghost, T(-1.0));
       T qx = viscous_stress(ri, up(u_pa, 1), u_pa, dn(u_pa, 1), dn(u_pa, 2));
-      const T qx_m = from_lane(qx, mc_lane);
-      if (cell_ghost) qx = qx_m;
+      qx = mirror_fix(xedge, qx, mc_lane, cell_ghost, T(1.0));
       const T mx_av = accelerate(mx_pa, dt, qx, up(qx, 1), cdx_q);
       const T uav = velocity(mx_av, rex);
       if (own_face) {

@@ -72,6 +72,14 @@ def lib():
         "flowhip_memset_zero": (i, [vp, sz]),
         "flowhip_copy": (i, [vp, vp, sz, i]),
         "flowhip_device_name": (i, [ctypes.c_char_p, i]),
+        "flowhip_f32_alloc": (i, [ctypes.POINTER(vp), sz]),
+        "flowhip_f32_free": (None, [vp]),
+        "flowhip_f32_zero": (i, [vp, sz]),
+        "flowhip_f32_copy": (i, [vp, vp, sz, i]),
+        "flowhip_f32_narrow": (i, [vp, vp, sz]),
+        "flowhip_f32_widen": (i, [vp, vp, sz]),
+        "flowhip_f32_boundary": (i, [mp, vp, i, i, i]),
+        "flowhip_run_f32": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp]),
         "allocate_data": (sz, [ctypes.POINTER(vp), sz]),
         "deallocate_data": (None, [vp]),
         "handle_boundary_2d": (None, [i, i, mp, vp, i, i]),
@@ -240,6 +248,74 @@ class DeviceState:
         steps = lib().flowhip_run(self.mesh.ptr, self.array_ptrs(), niters, tt0,
                                   1 if pre_timestep else 0, _ptr(dt),
                                   _ptr(mass), _ptr(energy), ctypes.byref(wall))
+        return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
+                energy[:steps], wall.value)
+
+
+class DeviceState32:
+    """The FP32-state precision form: the same 16 arrays as FP32 device
+    arrays of (L+1)^2 floats (flowhip_f32_alloc)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.n = (mesh.nx + 1) * (mesh.ny + 1)
+        self.ptrs = {}
+        for k in ARRAY_NAMES:
+            p = ctypes.c_void_p()
+            lib().flowhip_f32_alloc(ctypes.byref(p), self.n)
+            self.ptrs[k] = p.value
+
+    def close(self):
+        for k, p in list(self.ptrs.items()):
+            lib().flowhip_f32_free(ctypes.c_void_p(p))
+        self.ptrs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, name, host):
+        """Round host values (any float type) to FP32 and copy them in."""
+        host = np.ascontiguousarray(host, dtype=np.float64).ravel()
+        buf = np.zeros(self.n, dtype=np.float32)
+        buf[: host.size] = host.astype(np.float32)
+        lib().flowhip_f32_copy(ctypes.c_void_p(self.ptrs[name]),
+                               buf.ctypes.data_as(ctypes.c_void_p), self.n, SEND)
+
+    def zero(self, name):
+        lib().flowhip_f32_zero(ctypes.c_void_p(self.ptrs[name]), self.n)
+
+    def download(self, name):
+        """The array widened to float64."""
+        buf = np.empty(self.n, dtype=np.float32)
+        lib().flowhip_f32_copy(ctypes.c_void_p(self.ptrs[name]),
+                               buf.ctypes.data_as(ctypes.c_void_p), self.n, RECV)
+        return buf.astype(np.float64)
+
+    def download_all(self):
+        return {k: self.download(k) for k in ARRAY_NAMES}
+
+    def reflect_initial(self):
+        m = self.mesh
+        for k in ("rho", "e"):
+            lib().flowhip_f32_boundary(m.ptr, ctypes.c_void_p(self.ptrs[k]),
+                                       m.nx, m.ny, NO_INVERT)
+
+    def array_ptrs(self):
+        return (ctypes.c_void_p * 16)(*[self.ptrs[k] for k in ARRAY_NAMES])
+
+    def run(self, niters, tt0=0, pre_timestep=True):
+        """main.c:64-115 on FP32 state (flowhip_run_f32). Returns (steps,
+        dt_pairs, mass, energy, wall_seconds); scalars are FP64."""
+        dt = np.zeros(2 * (niters + 1))
+        mass = np.zeros(max(niters, 1))
+        energy = np.zeros(max(niters, 1))
+        wall = ctypes.c_double()
+        steps = lib().flowhip_run_f32(self.mesh.ptr, self.array_ptrs(), niters,
+                                      tt0, 1 if pre_timestep else 0, _ptr(dt),
+                                      _ptr(mass), _ptr(energy), ctypes.byref(wall))
         return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
                 energy[:steps], wall.value)
 

@@ -58,6 +58,28 @@ int flowhip_run(Mesh* mesh, double** arrays, int niters, int tt0,
                 int pre_timestep, double* dt_pairs, double* mass,
                 double* energy, double* wall_seconds);
 
+/* ---- FP32-state precision form ----
+ * The 16 arrays of solve_hydro_2d (same roles, layouts and (L+1)^2 sizes)
+ * held as FP32 device arrays; the march kernels run on them in FP32. The
+ * CFL candidates and the dt reduction, the mass/energy totals (compensated
+ * sums) and the EOS at near-vacuum cells (rho < 1e-6) are FP64; the dt pair
+ * is FP64 on the device. Geometry is the mesh's, rounded to FP32. */
+int flowhip_f32_alloc(float** buf, size_t len);  /* zeroed device floats */
+void flowhip_f32_free(float* buf);
+int flowhip_f32_zero(float* buf, size_t len);
+/* Host<->device copies of len floats (dir: SEND = host to device). */
+int flowhip_f32_copy(float* dev, float* host, size_t len, int dir);
+/* Device-to-device conversions (round to nearest when narrowing). */
+int flowhip_f32_narrow(float* dst, const double* src, size_t len);
+int flowhip_f32_widen(double* dst, const float* src, size_t len);
+/* handle_boundary_2d on an FP32 device array. */
+int flowhip_f32_boundary(Mesh* mesh, float* arr, int nxp, int nyp, int invert);
+/* flowhip_run on FP32 state: arrays in solve_hydro_2d order; dt pairs and
+ * totals are FP64. */
+int flowhip_run_f32(Mesh* mesh, float** arrays, int niters, int tt0,
+                    int pre_timestep, double* dt_pairs, double* mass,
+                    double* energy, double* wall_seconds);
+
 /* Device helpers for callers that manage their own buffers. */
 int flowhip_device_sync(void);
 int flowhip_memset_zero(double* dev, size_t len);
