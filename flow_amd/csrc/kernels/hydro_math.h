@@ -21,6 +21,19 @@ constexpr double kCM = 1.5 / kCT;     /* flow_interface.h:9 */
 __device__ __forceinline__ double vabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
 
+/* a / b. FP64: the IEEE quotient, as omp3 computes it. FP32 state: a times
+ * the hardware reciprocal of b (within 1 ulp of the reciprocal); the
+ * precision form trades the last bits of each quotient for the ten-odd
+ * instructions of a correctly rounded FP32 division. */
+template <typename T>
+__device__ __forceinline__ T quot(T a, T b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float quot<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
 /* arch min(a, b) = a < b ? a : b */
 template <typename T>
 __device__ __forceinline__ T lesser(T a, T b) {
@@ -56,7 +69,7 @@ __device__ __forceinline__ float eos_pressure<float>(float rho, float e) {
 template <typename T>
 __device__ __forceinline__ T rho_edge_x(T rho_c, T rho_w, T cdx_c, T cdx_w,
                                         T cdy, T edx) {
-  return (rho_c * cdx_c * cdy + rho_w * cdx_w * cdy) / (T(2.0) * edx * cdy);
+  return quot(rho_c * cdx_c * cdy + rho_w * cdx_w * cdy, T(2.0) * edx * cdy);
 }
 
 /* Edge-centred density on a y face between cells i-1 (s) and i (c):
@@ -64,20 +77,20 @@ __device__ __forceinline__ T rho_edge_x(T rho_c, T rho_w, T cdx_c, T cdx_w,
 template <typename T>
 __device__ __forceinline__ T rho_edge_y(T rho_c, T rho_s, T cdx, T cdy_c,
                                         T cdy_s, T edy) {
-  return (rho_c * cdx * cdy_c + rho_s * cdx * cdy_s) / (T(2.0) * cdx * edy);
+  return quot(rho_c * cdx * cdy_c + rho_s * cdx * cdy_s, T(2.0) * cdx * edy);
 }
 
 /* Velocity from momentum with the zero-density guard, omp3/flow.c:144-150. */
 template <typename T>
 __device__ __forceinline__ T velocity(T mom, T rho_edge) {
-  return (rho_edge == T(0.0)) ? T(0.0) : mom / rho_edge;
+  return (rho_edge == T(0.0)) ? T(0.0) : quot(mom, rho_edge);
 }
 
 /* mom -= dt * (a - b) / d: the pressure (omp3/flow.c:126-131) and viscous
  * (:216-219) accelerations. */
 template <typename T>
 __device__ __forceinline__ T accelerate(T mom, T dt, T a, T b, T d) {
-  return mom - dt * (a - b) / d;
+  return mom - quot(dt * (a - b), d);
 }
 
 /* mom -= dt_h * (a - b) / (d1 * d2): the momentum advection updates
@@ -85,7 +98,7 @@ __device__ __forceinline__ T accelerate(T mom, T dt, T a, T b, T d) {
 template <typename T>
 __device__ __forceinline__ T advect_update(T mom, T dt_h, T a, T b, T d1,
                                            T d2) {
-  return mom - dt_h * (a - b) / (d1 * d2);
+  return mom - quot(dt_h * (a - b), d1 * d2);
 }
 
 /* Artificial viscous stress from the four face velocities w0..w3 around a
@@ -106,13 +119,13 @@ template <typename T>
 __device__ __forceinline__ T shock_heating(T e, T P, T rho, T qxx, T qyy,
                                            T u0, T u1, T v0, T v1, T cdx,
                                            T cdy, T dt_h) {
-  const T dvx = (u1 - u0) / cdx;
-  const T dvy = (v1 - v0) / cdy;
+  const T dvx = quot(u1 - u0, cdx);
+  const T dvy = quot(v1 - v0, cdy);
   const T div_dt = (dvx + dvy) * dt_h;
-  const T e_q = e - dt_h * (qxx * dvx + qyy * dvy) / rho;
-  const T rho_c = rho / (T(1.0) + div_dt);
-  const T e_c = e_q - (P * div_dt) / rho;
-  const T work = T(0.5) * div_dt * (P + T(kGam - 1.0) * e_c * rho_c) / rho;
+  const T e_q = e - quot(dt_h * (qxx * dvx + qyy * dvy), rho);
+  const T rho_c = quot(rho, T(1.0) + div_dt);
+  const T e_c = e_q - quot(P * div_dt, rho);
+  const T work = quot(T(0.5) * div_dt * (P + T(kGam - 1.0) * e_c * rho_c), rho);
   return (rho == T(0.0)) ? T(0.0) : e_q - work;
 }
 
@@ -178,13 +191,13 @@ __device__ __forceinline__ FaceFlux<T> mass_energy_flux(
    * 0, as in the reference's if (density_diff). */
   const T drho = (rp - rm);
   const T sel = kYSweep ? v0 : v_tc;
-  const T r = ((sel >= T(0.0)) ? (rm - rmm) : (rpp - rp)) / drho;
+  const T r = quot((sel >= T(0.0)) ? (rm - rmm) : (rpp - rp), drho);
   const T limiter =
-      (drho != T(0.0)) ? (r + vabs(r)) / (T(1.0) + vabs(r)) : T(0.0);
+      (drho != T(0.0)) ? quot(r + vabs(r), T(1.0) + vabs(r)) : T(0.0);
   const T rho_up = (v_tc >= T(0.0)) ? rm : rp;
   FaceFlux<T> out;
   out.mass = (v_tc * rho_up + T(0.5) * vabs(v_tc) *
-                                  (T(1.0) - vabs((v_tc * dt_h) / cd_hi)) *
+                                  (T(1.0) - vabs(quot(v_tc * dt_h, cd_hi))) *
                                   limiter * drho);
   const T e_face = mc_upwind(emm, em, ep, epp, inv, v_tc, v_tc > T(0.0),
                              cd_lo, cd_hi, dt_h);
@@ -205,10 +218,10 @@ __device__ __forceinline__ CellState<T> sweep_update(
     T rho, T rho_old, T e, T f_lo, T f_hi, T ef_lo, T ef_hi, T et_lo, T et_hi,
     T cdx, T cdy, T dt_h, bool first) {
   CellState<T> out;
-  out.rho = rho - dt_h * (et_hi * f_hi - et_lo * f_lo) / (cdx * cdy);
-  const T rho_e = (rho_old * e - (dt_h * (ef_hi - ef_lo)) / (cdx * cdy));
-  out.e = first ? ((rho_old == T(0.0)) ? T(0.0) : rho_e / rho_old)
-                : ((out.rho == T(0.0)) ? T(0.0) : rho_e / out.rho);
+  out.rho = rho - quot(dt_h * (et_hi * f_hi - et_lo * f_lo), cdx * cdy);
+  const T rho_e = (rho_old * e - quot(dt_h * (ef_hi - ef_lo), cdx * cdy));
+  out.e = first ? ((rho_old == T(0.0)) ? T(0.0) : quot(rho_e, rho_old))
+                : ((out.rho == T(0.0)) ? T(0.0) : quot(rho_e, out.rho));
   return out;
 }
 

@@ -44,6 +44,11 @@ struct MMassIO {
 };
 
 constexpr int kMNT = 256;            /* threads per block: 4 independent waves */
+/* Minimum resident waves per SIMD the register allocation must allow
+ * (the second __launch_bounds__ argument); 1 leaves the compiler free. */
+#ifndef FLOWHIP_MARCH_WAVES
+#define FLOWHIP_MARCH_WAVES 1
+#endif
 constexpr int kMWaves = kMNT / kWave;
 
 struct MarchList {
@@ -183,7 +188,7 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
                                                         MarchList ml) {
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(kMNT) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
                                                         MarchList ml) {
@@ -481,7 +486,7 @@ __device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
  * a window of u1 rows r-1 .. r+2. Ghost rows are the mirror row's values
  * (u is copied unsigned across the south/north walls). */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
@@ -577,7 +582,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
  * from a window of u0 rows r-1 .. r+2; ghost faces hold the negated mirror
  * face's u1 (INVERT_X) and ghost cells the mirror cell's flux. */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
@@ -661,7 +666,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
  * of v1 rows g-1 .. g+2; a ghost cell row's flux is its mirror row's. io.u
  * is u after the x sub-updates. */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
@@ -791,7 +796,7 @@ __global__ __launch_bounds__(kMNT) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
  * row (ghost columns: the mirror column, unsigned) and the corner fluxes
  * across the lanes. io.u is u after the x sub-updates. */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
@@ -908,7 +913,7 @@ struct MLagIO {
  * viscous acceleration on faces i and i+1. Ghost faces hold the negated
  * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
 template <typename T>
-__global__ __launch_bounds__(kMNT) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
                                                            const double* dtp,
                                                            FkReduce red, double max_dt,
                                                            int first_step, MarchList ml) {
