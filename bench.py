@@ -67,6 +67,12 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true",
                    help="skip the FP32-state precision form")
+    p.add_argument("--no-ensemble", action="store_true",
+                   help="skip the batching form")
+    p.add_argument("--ens-count", type=int, default=256)
+    p.add_argument("--ens-steps", type=int, default=300)
+    p.add_argument("--ens-dt", type=float, default=1.0e-4,
+                   help=".params dt of every ensemble instance")
     return p.parse_args()
 
 
@@ -169,6 +175,44 @@ def fp32_leg(args, n, rho0, e0, ref_state, ref_dts, ref_mass, ref_energy,
         "comparator": "the FP64 GPU state after the same steps (bitwise equal "
                       "to the oracle: parity_vs_oracle)",
     }
+
+
+def ensemble_leg(args):
+    """The batching form on the seeded ensemble workload: args.ens_count
+    independent Sod/blast instances of sides log-uniform in [64, 1024],
+    every instance its own dt, args.ens_steps steps, FP64 and FP32 state.
+    The autotuner picks the chunk height and work order first (from the
+    initial state, which it restores); steps are replayed from hipGraphs."""
+    import flow_amd
+    from flow_amd import problems
+    specs = problems.ensemble_specs(count=args.ens_count)
+    sizes = [sp[0] for sp in specs]
+    cells = float(sum(n * n for n in sizes))
+    out = {"instances": len(sizes), "steps": args.ens_steps, "cells": cells,
+           "sizes": {"min": min(sizes), "max": max(sizes)},
+           "workload": f"{len(sizes)} seeded Sod/blast instances, sides log-U[64,1024], "
+                       f"split U[0.3,0.7], pressure ratio log-U[10,1000], own dt each"}
+    for label, fp32 in (("fp64", False), ("fp32", True)):
+        ens = flow_amd.Ensemble(sizes, dt=args.ens_dt, fp32=fp32)
+        for k, sp in enumerate(specs):
+            ens.set_state(k, *problems.ensemble_instance(*sp))
+        (rows, order), tuned_ms = ens.autotune(trial_steps=3)
+        ens.set_graphs(True)
+        dts, mass, energy, wall = ens.run(args.ens_steps)
+        ok = bool(np.all(np.isfinite(dts)) and np.all(dts[:, :, 0] > 0))
+        out[label] = {
+            "solves_per_s": len(sizes) / wall,
+            "unit": "ensemble solves/s (one solve = one instance advanced "
+                    f"{args.ens_steps} steps)",
+            "cell_updates_per_s": cells * args.ens_steps / wall,
+            "wall_s": wall, "ms_per_step": wall / args.ens_steps * 1e3,
+            "autotuned": {"chunk_rows": rows, "work_order": order,
+                          "trial_ms_per_step": tuned_ms},
+            "finite": ok,
+            "dt_spread": [float(dts[:, -1, 0].min()), float(dts[:, -1, 0].max())],
+        }
+        ens.close()
+    return out
 
 
 def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
@@ -306,6 +350,7 @@ def main():
 
     st.close()
     mesh.close()
+    result["ensemble"] = None if args.no_ensemble else ensemble_leg(args)
     print(json.dumps(result), flush=True)
 
 

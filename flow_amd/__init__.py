@@ -8,7 +8,7 @@ library has not been built.
 """
 from .native import (  # noqa: F401
     ARRAY_NAMES, FUSED, MARCH, STAGEWISE, Mesh, MeshStruct, DeviceState,
-    DeviceState32, lib,
+    DeviceState32, Ensemble, lib,
     library_path,
 )
 from . import problems  # noqa: F401

@@ -1,1144 +1,10 @@
 /*
- * march.hip -- row-marching forms of the fused kernels.
- *
- * One wave owns a strip of W columns and marches down a chunk of H rows.
- * Along x a wave covers the strip plus its stencil halo in its 64 lanes, and
- * x neighbours come from cross-lane shuffles; along y every lane keeps the
- * rows its column still needs in a register window, so nothing along y is
- * recomputed and no block barrier is taken. Input rows are loaded one step
- * ahead of use. Waves of a block are independent.
- *
- * The arithmetic is the same pointwise code (hydro_math.h) in the same
- * order, so results are bitwise those of the tile kernels and of omp3;
- * ghost rows and columns are the mirrored values handle_boundary_2d would
- * copy there.
+ * march.hip -- launchers of the row-marching kernels on one mesh (FP64 and
+ * the FP32-state form). The kernels are in march.h.
  */
-#include <type_traits>
-
-#include "reduce_common.h"
-#include "tile.h"
+#include "march.h"
 
 namespace flowhip {
-
-template <typename T>
-struct MGeo {
-  int nx, ny;
-  const T* __restrict__ edx;
-  const T* __restrict__ edy;
-  const T* __restrict__ cdx;
-  const T* __restrict__ cdy;
-  const T* __restrict__ iedx;
-  const T* __restrict__ iedy;
-};
-
-template <typename T>
-struct MMassIO {
-  const T* __restrict__ u0;
-  const T* __restrict__ v0;
-  const T* __restrict__ rho0;
-  const T* __restrict__ e1;
-  T* __restrict__ rho;
-  T* __restrict__ e;
-  T* __restrict__ F_x;
-  T* __restrict__ F_y;
-};
-
-constexpr int kMNT = 256;            /* threads per block: 4 independent waves */
-/* Minimum resident waves per SIMD the register allocation must allow
- * (the second __launch_bounds__ argument); 1 leaves the compiler free. */
-#ifndef FLOWHIP_MARCH_WAVES
-#define FLOWHIP_MARCH_WAVES 1
-#endif
-constexpr int kMWaves = kMNT / kWave;
-
-struct MarchList {
-  int W, H;           /* strip width (output columns), chunk height (rows) */
-  int nstrips, nchunks, nitems;
-};
-
-inline MarchList make_march(int nx, int ny, int W, int H) {
-  MarchList m;
-  m.W = W;
-  m.H = H;
-  m.nstrips = (nx - 2 * kPad + W - 1) / W;
-  m.nchunks = (ny - 2 * kPad + H - 1) / H;
-  m.nitems = m.nstrips * m.nchunks;
-  return m;
-}
-
-/* Whole-wave shifts by one lane as DPP moves (wave_shr:1 / wave_shl:1):
- * VALU operations on registers, with none of the LDS round trip of a
- * ds_bpermute. Lane 0 of a shift up and lane 63 of a shift down keep their
- * own value; no kernel reads those lanes' results. */
-constexpr int kDppWaveShl1 = 0x130; /* lane i <- lane i+1 */
-constexpr int kDppWaveShr1 = 0x138; /* lane i <- lane i-1 */
-
-template <int CTRL>
-__device__ __forceinline__ int dpp_move(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = dpp_move<CTRL>((int)b);
-  const int hi = dpp_move<CTRL>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
-}
-
-/* value of the lane d below (up) or above (dn), d = 1 or 2 */
-template <typename T>
-__device__ __forceinline__ T up(T v, int d) {
-  const T v1 = dpp_move<kDppWaveShr1>(v);
-  return d == 1 ? v1 : dpp_move<kDppWaveShr1>(v1);
-}
-template <typename T>
-__device__ __forceinline__ T dn(T v, int d) {
-  const T v1 = dpp_move<kDppWaveShl1>(v);
-  return d == 1 ? v1 : dpp_move<kDppWaveShl1>(v1);
-}
-
-
-/* ---- per-row geometry of a chunk held across the lanes ----
- * Lane k holds rows base+k and base+64+k; row(k) broadcasts one row's value
- * with v_readlane (k is uniform), so the row loop issues no memory load for
- * geometry. Chunks are at most kMaxChunkRows rows. */
-constexpr int kMaxChunkRows = 120;
-
-__device__ __forceinline__ double lane_value(double v, int k) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, k);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float lane_value(float v, int k) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-
-template <typename T>
-struct RowGeo {
-  T a, b;
-  int base;
-  __device__ RowGeo(const T* arr, int base_row, int last_row, int lane) : base(base_row) {
-    /* rows outside [0, last_row] are never asked for; clamp the loads */
-    a = arr[max(0, min(base_row + lane, last_row))];
-    b = arr[max(0, min(base_row + kWave + lane, last_row))];
-  }
-  /* value at row r (uniform, base <= r < base + 128) */
-  __device__ __forceinline__ T operator()(int r) const {
-    const int k = r - base;
-    return k < kWave ? lane_value(a, k) : lane_value(b, k - kWave);
-  }
-};
-
-/* ---- boundary-strip writers (ghost writes only in edge items) ---- */
-template <typename T>
-__device__ __forceinline__ void mput_cell(bool edge, T* a, int nx, int ny, int i,
-                                          int j, T v) {
-  a[i * nx + j] = v;
-  if (!edge) return;
-  if (j == kPad) a[i * nx + kPad - 1] = v;
-  if (j == kPad + 1) a[i * nx + kPad - 2] = v;
-  if (j == nx - kPad - 1) a[i * nx + nx - kPad] = v;
-  if (j == nx - kPad - 2) a[i * nx + nx - kPad + 1] = v;
-  if (i == kPad) a[(kPad - 1) * nx + j] = v;
-  if (i == kPad + 1) a[(kPad - 2) * nx + j] = v;
-  if (i == ny - kPad - 1) a[(ny - kPad) * nx + j] = v;
-  if (i == ny - kPad - 2) a[(ny - kPad + 1) * nx + j] = v;
-}
-template <typename T>
-__device__ __forceinline__ void mput_xface(bool edge, T* a, int nx, int ny, int i,
-                                           int f, T v, T sx) {
-  const int s = nx + 1;
-  a[i * s + f] = v;
-  if (!edge) return;
-  if (f == kPad) a[i * s + kPad - 1] = sx * v;
-  if (f == kPad + 1) a[i * s + kPad - 2] = sx * v;
-  if (f == nx - kPad) a[i * s + nx - kPad + 1] = sx * v;
-  if (f == nx - kPad - 1) a[i * s + nx - kPad + 2] = sx * v;
-  if (i == kPad) a[(kPad - 1) * s + f] = v;
-  if (i == kPad + 1) a[(kPad - 2) * s + f] = v;
-  if (i == ny - kPad - 1) a[(ny - kPad) * s + f] = v;
-  if (i == ny - kPad - 2) a[(ny - kPad + 1) * s + f] = v;
-}
-template <typename T>
-__device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int g,
-                                           int j, T v, T sy) {
-  a[g * nx + j] = v;
-  if (!edge) return;
-  if (g == kPad) a[(kPad - 1) * nx + j] = sy * v;
-  if (g == kPad + 1) a[(kPad - 2) * nx + j] = sy * v;
-  if (g == ny - kPad) a[(ny - kPad + 1) * nx + j] = sy * v;
-  if (g == ny - kPad - 1) a[(ny - kPad + 2) * nx + j] = sy * v;
-  if (j == kPad) a[g * nx + kPad - 1] = v;
-  if (j == kPad + 1) a[g * nx + kPad - 2] = v;
-  if (j == nx - kPad - 1) a[g * nx + nx - kPad] = v;
-  if (j == nx - kPad - 2) a[g * nx + nx - kPad + 1] = v;
-}
-
-/* ======================================================================= */
-/* mass and energy, x sweep first (even tt)                                 */
-/* ======================================================================= */
-
-/* Lane l holds column q = x0 - 2 + l. Per row r: the x fluxes at faces
- * [x0, x1] and the first (x) update of cells [x0, x1); per lane a window of
- * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
- * update of row r-2. */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
-                                                        const double* dtp,
-                                                        FkReduce red,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const int q = x0 - 2 + lane;
-    const bool inq = q < nx;                     /* column exists */
-    const bool face = q >= x0 && q <= x1;        /* lane computes face q */
-    const bool own = q >= x0 && q < x1;          /* lane outputs cell q */
-    const bool own_face = face && (q < x1 || x1 == nx - kPad);
-    const int qc = inq ? q : nx - 1;             /* a valid column for loads */
-    const int qf = q <= nx ? q : nx;             /* a valid x face for loads */
-    /* per-lane geometry, loaded once per item: no load inside the row loop
-     * other than the next row's prefetch */
-    const T cdx_q = g.cdx[qc], edx_q = g.edx[qc], edx_q1 = g.edx[qc + 1];
-    const int qg = face ? q : x0;
-    const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    /* rows of the x-updated fields for this lane's column: w[0] oldest */
-    T wr[4], we[4];
-    T vy[3];   /* v0 at faces r-2 .. r */
-    T r0[3];   /* rho0 at rows r-2 .. r */
-    T fy_prev = T(0), efy_prev = T(0);
-    const bool south = (y0 == kPad);
-    const int rb = south ? kPad : y0 - 2;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) wr[k] = we[k] = T(0);
-    vy[0] = io.v0[(rb - 2) * nx + qc];
-    vy[1] = io.v0[(rb - 1) * nx + qc];
-    r0[0] = io.rho0[(rb - 2) * nx + qc];
-    r0[1] = io.rho0[(rb - 1) * nx + qc];
-    /* row rb's inputs; later rows are loaded one step ahead */
-    T a_r = io.rho0[rb * nx + qc], a_e = io.e1[rb * nx + qc];
-    T a_u = io.u0[rb * sx + qf], a_v = io.v0[rb * nx + qc];
-    for (int r = rb; r <= y1 + 1; ++r) {
-      const bool ghost_row = r >= ny - kPad;
-      /* prefetch the next row */
-      const int rn = min(r + 1, ny - kPad - 1);
-      const T n_r = io.rho0[rn * nx + qc], n_e = io.e1[rn * nx + qc];
-      const T n_u = io.u0[rn * sx + qf];
-      const T n_v = io.v0[min(r + 1, ny) * nx + qc];
-      vy[2] = a_v;
-      r0[2] = a_r;
-
-      T xr, xe;
-      if (!ghost_row) {
-        /* x fluxes at face q from the row in the lanes around it */
-        const T rm2 = up(a_r, 2), rm1 = up(a_r, 1), rp1 = dn(a_r, 1);
-        const T em2 = up(a_e, 2), em1 = up(a_e, 1), ep1 = dn(a_e, 1);
-        const T um1 = up(a_u, 1), up1 = dn(a_u, 1);
-        FaceFlux<T> fl{T(0), T(0)};
-        if (face)
-          fl = mass_energy_flux<T, false>(um1, a_u, up1, rm2, rm1, a_r, rp1, em2, em1,
-                                          a_e, ep1, iedx_q, cdx_qm1, cdx_q,
-                                          edy(r), dt, dt_h);
-        if (own_face && r >= y0 && r < y1)
-          mput_xface(edge, io.F_x, nx, ny, r, q, fl.mass, T(-1.0));
-        const T f_hi = dn(fl.mass, 1), ef_hi = dn(fl.energy, 1);
-        const CellState<T> st =
-            sweep_update(a_r, a_r, a_e, fl.mass, f_hi, fl.energy, ef_hi, edy(r),
-                         edy(r + 1), cdx_q, cdy(r), dt_h, true);
-        xr = st.rho;
-        xe = st.e;
-      } else {
-        /* ghost row past the north wall: the mirror row's x update */
-        xr = (r == ny - kPad) ? wr[3] : wr[1];
-        xe = (r == ny - kPad) ? we[3] : we[1];
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        wr[k] = wr[k + 1];
-        we[k] = we[k + 1];
-      }
-      wr[3] = xr;
-      we[3] = xe;
-      if (south && r == kPad + 1) {
-        /* ghost rows below the south wall mirror rows pad, pad+1 */
-        wr[0] = wr[3];
-        we[0] = we[3];
-        wr[1] = wr[2];
-        we[1] = we[2];
-      }
-      /* y flux at face r-1 from rows r-3 .. r */
-      const int gf = r - 1;
-      if (gf >= y0 && gf <= y1) {
-        const FaceFlux<T> fl = mass_energy_flux<T, true>(
-            vy[0], vy[1], vy[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2],
-            we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_q, dt, dt_h);
-        if (own && (gf < y1 || y1 == ny - kPad))
-          mput_yface(edge, io.F_y, nx, ny, gf, q, fl.mass, T(-1.0));
-        /* y update of row r-2 */
-        const int i = r - 2;
-        if (i >= y0 && own) {
-          const CellState<T> st =
-              sweep_update(wr[1], r0[0], we[1], fy_prev, fl.mass, efy_prev,
-                           fl.energy, edx_q, edx_q1, cdx_q, cdy(i), dt_h, false);
-          mput_cell(edge, io.rho, nx, ny, i, q, st.rho);
-          mput_cell(edge, io.e, nx, ny, i, q, st.e);
-          ms.add(double(st.rho));
-          es.add(double(st.rho) * double(st.e));
-        }
-        fy_prev = fl.mass;
-        efy_prev = fl.energy;
-      }
-      vy[0] = vy[1];
-      vy[1] = vy[2];
-      r0[0] = r0[1];
-      r0[1] = r0[2];
-      a_r = n_r;
-      a_e = n_e;
-      a_u = n_u;
-      a_v = n_v;
-    }
-  }
-  finish_sums<kMNT>(ms, es, red);
-}
-
-/* ======================================================================= */
-/* mass and energy, y sweep first (odd tt)                                  */
-/* ======================================================================= */
-
-/* Lane l holds column q = x0 - 2 + l and runs the y sweep at its mirror
- * column (the same column inside the domain). Per row r: the y flux at face
- * r+1 and the first (y) update of row r; then across the lanes the x fluxes
- * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
-                                                        const double* dtp,
-                                                        FkReduce red,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const int q = x0 - 2 + lane;
-    const bool inq = q < nx;
-    const bool face = q >= x0 && q <= x1;
-    const bool own = q >= x0 && q < x1;
-    const bool own_face = face && (q < x1 || x1 == nx - kPad);
-    /* the y sweep of a ghost column is that of its mirror column */
-    const int qm = inq ? mirror(q, nx) : nx - kPad - 1;
-    const int qc = inq ? q : nx - 1;
-    const int qf = q <= nx ? q : nx;
-    const T cdx_m = g.cdx[qm], edx_m = g.edx[qm], edx_m1 = g.edx[qm + 1];
-    const T cdx_q = g.cdx[qc];
-    const int qg = face ? q : x0;
-    const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2 of column qm */
-    T wr[4], we[4], wv[3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      wr[k] = io.rho0[(y0 - 2 + k) * nx + qm];
-      we[k] = io.e1[(y0 - 2 + k) * nx + qm];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) wv[k] = io.v0[(y0 - 1 + k) * nx + qm];
-    /* y flux at face y0 (rows y0-2 .. y0+1) */
-    FaceFlux<T> fprev = mass_energy_flux<T, true>(
-        wv[0], wv[1], wv[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2], we[3],
-        iedy(y0), cdy(y0 - 1), cdy(y0), edx_m, dt, dt_h);
-    if (own && q == qm) mput_yface(edge, io.F_y, nx, ny, y0, q, fprev.mass, T(-1.0));
-    /* shift: windows now hold rows r-2 .. r+1 for r = y0; load row r+2 */
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      wr[k] = wr[k + 1];
-      we[k] = we[k + 1];
-    }
-    wv[0] = wv[1];
-    wv[1] = wv[2];
-    T n_r = io.rho0[(y0 + 2) * nx + qm], n_e = io.e1[(y0 + 2) * nx + qm];
-    T n_v = io.v0[(y0 + 2) * nx + qm];
-    T a_u = io.u0[y0 * sx + qf];
-    T a_r0 = io.rho0[y0 * nx + qc];
-    for (int r = y0; r < y1; ++r) {
-      wr[3] = n_r;
-      we[3] = n_e;
-      wv[2] = n_v;
-      const T c_u = a_u, c_r0 = a_r0;
-      /* prefetch the next row's inputs */
-      {
-        const int rn = min(r + 3, ny);
-        n_r = io.rho0[rn * nx + qm];
-        n_e = io.e1[rn * nx + qm];
-        n_v = io.v0[rn * nx + qm];
-        const int rr = min(r + 1, ny - 1);
-        a_u = io.u0[rr * sx + qf];
-        a_r0 = io.rho0[rr * nx + qc];
-      }
-      /* y flux at face r+1 (rows r-1 .. r+2) and the y update of row r */
-      const int gf = r + 1;
-      const FaceFlux<T> fl = mass_energy_flux<T, true>(
-          wv[0], wv[1], wv[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2],
-          we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_m, dt, dt_h);
-      if (own && q == qm && (gf < y1 || y1 == ny - kPad))
-        mput_yface(edge, io.F_y, nx, ny, gf, q, fl.mass, T(-1.0));
-      const CellState<T> ys =
-          sweep_update(wr[1], wr[1], we[1], fprev.mass, fl.mass, fprev.energy,
-                       fl.energy, edx_m, edx_m1, cdx_m, cdy(r), dt_h, true);
-      fprev = fl;
-      /* x fluxes at face q of the y-updated row, then the x update */
-      const T rm2 = up(ys.rho, 2), rm1 = up(ys.rho, 1), rp1 = dn(ys.rho, 1);
-      const T em2 = up(ys.e, 2), em1 = up(ys.e, 1), ep1 = dn(ys.e, 1);
-      const T um1 = up(c_u, 1), up1 = dn(c_u, 1);
-      FaceFlux<T> fx{T(0), T(0)};
-      if (face)
-        fx = mass_energy_flux<T, false>(um1, c_u, up1, rm2, rm1, ys.rho, rp1, em2, em1,
-                                        ys.e, ep1, iedx_q, cdx_qm1, cdx_q,
-                                        edy(r), dt, dt_h);
-      if (own_face) mput_xface(edge, io.F_x, nx, ny, r, q, fx.mass, T(-1.0));
-      const T f_hi = dn(fx.mass, 1), ef_hi = dn(fx.energy, 1);
-      if (own) {
-        const CellState<T> st =
-            sweep_update(ys.rho, c_r0, ys.e, fx.mass, f_hi, fx.energy, ef_hi,
-                         edy(r), edy(r + 1), cdx_q, cdy(r), dt_h, false);
-        mput_cell(edge, io.rho, nx, ny, r, q, st.rho);
-        mput_cell(edge, io.e, nx, ny, r, q, st.e);
-        ms.add(double(st.rho));
-        es.add(double(st.rho) * double(st.e));
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        wr[k] = wr[k + 1];
-        we[k] = we[k + 1];
-      }
-      wv[0] = wv[1];
-      wv[1] = wv[2];
-    }
-  }
-  finish_sums<kMNT>(ms, es, red);
-}
-
-
-/* ======================================================================= */
-/* momentum                                                                */
-/* ======================================================================= */
-
-template <typename T>
-struct MMomIO {
-  const T* __restrict__ u;    /* x velocity read (u0 for x, u1 for y) */
-  const T* __restrict__ v0;   /* y velocity after the viscous step */
-  const T* __restrict__ F_x;
-  const T* __restrict__ F_y;
-  const T* __restrict__ rho;  /* advected density */
-  const T* __restrict__ mom1; /* momentum after the viscous step */
-  T* __restrict__ mom;        /* advected momentum */
-  T* __restrict__ vel;        /* recomputed velocity, + ghosts */
-};
-
-/* value of v at lane src of the wave */
-template <typename T>
-__device__ __forceinline__ T from_lane(T v, int src) {
-  return __shfl(v, src, kWave);
-}
-
-/* A ghost lane's value: sign times its mirror lane's. Only strips within
- * three columns of a wall hold ghost lanes (xedge, uniform over the wave),
- * so only they pay for the cross-lane permute. */
-template <typename T>
-__device__ __forceinline__ T mirror_fix(bool xedge, T v, int src, bool ghost, T sign) {
-  if (!xedge) return v;
-  const T m = from_lane(v, src);
-  return ghost ? sign * m : v;
-}
-
-/* lane holding the mirror of index q (lane holds q), kept inside the wave */
-__device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
-  return min(max(lane + (mirror(q, n) - q), 0), kWave - 1);
-}
-
-/* x momentum, odd tt (omp3/flow.c:570-619): x flux in x, u recompute, then
- * x flux in y. Lane l holds x face / cell column q = x0 - 2 + l. Per row k
- * the cell fluxes, the first sub-update and u1 are computed two rows ahead
- * of the output row r, which takes the corner fluxes at rows r and r+1 from
- * a window of u1 rows r-1 .. r+2. Ghost rows are the mirror row's values
- * (u is copied unsigned across the south/north walls). */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
-    const int q = x0 - 2 + lane;
-    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
-    const bool cell_ghost = is_ghost(q, nx);
-    const int mc_lane = mirror_lane(lane, q, nx);
-    const bool own = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
-    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-    const T iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    /* u1 and m1 of row i from that row's u0, F_x, mom1 and rho */
-    auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
-      const T up1 = dn(a_u, 1);
-      T ufx = momentum_flux(edy(i), a_fx, dn(a_fx, 1), up(a_u, 1), a_u, up1,
-                            dn(a_u, 2), iedx_q, T(0.5) * (a_u + up1), cdx_qm,
-                            cdx_q, dt_h);
-      ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
-      m1 = advect_update(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i));
-      const T rex = rho_edge_x(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q);
-      u1 = velocity(m1, rex);
-    };
-    /* x flux in y at corner row gf from u1 rows gf-2 .. gf+1 */
-    auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
-      const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
-      return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
-                           v_cell, cdy(gf - 1), cdy(gf), dt_h);
-    };
-
-    T wu[4], wm[3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int i = mirror(y0 - 2 + k, ny);
-      T m1, u1;
-      xrow(i, io.u[i * sx + qf], io.F_x[i * sx + qf], io.mom1[i * sx + qf],
-           io.rho[i * nx + qc], m1, u1);
-      wu[k] = u1;
-      if (k >= 2) wm[k - 2] = m1;
-    }
-    T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], io.v0[y0 * nx + qc],
-                       io.F_y[y0 * nx + qc]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
-    int ia = mirror(y0 + 2, ny);
-    T a_u = io.u[ia * sx + qf], a_fx = io.F_x[ia * sx + qf];
-    T a_m1 = io.mom1[ia * sx + qf], a_r = io.rho[ia * nx + qc];
-    T a_v = io.v0[(y0 + 1) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
-    for (int r = y0; r < y1; ++r) {
-      const T c_u = a_u, c_fx = a_fx, c_m1 = a_m1, c_r = a_r, c_v = a_v, c_fy = a_fy;
-      {
-        const int in = mirror(r + 3, ny), rv = min(r + 2, ny);
-        a_u = io.u[in * sx + qf];
-        a_fx = io.F_x[in * sx + qf];
-        a_m1 = io.mom1[in * sx + qf];
-        a_r = io.rho[in * nx + qc];
-        a_v = io.v0[rv * nx + qc];
-        a_fy = io.F_y[rv * nx + qc];
-      }
-      T m1, u1;
-      xrow(mirror(r + 2, ny), c_u, c_fx, c_m1, c_r, m1, u1);
-      wu[3] = u1;
-      wm[2] = m1;
-      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
-      if (own) {
-        io.mom[r * sx + q] = advect_update(wm[0], dt_h, uf, uf_prev, cdx_q, edy(r));
-        mput_xface(edge, io.vel, nx, ny, r, q, wu[1], T(-1.0));
-      }
-      uf_prev = uf;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
-      wm[0] = wm[1];
-      wm[1] = wm[2];
-    }
-  }
-}
-
-/* x momentum, even tt (omp3/flow.c:673-722): x flux in y, u recompute, then
- * x flux in x. Lane l holds x face / cell column q = x0 - 3 + l (strips of
- * 59 columns, so faces [x0-2, x1+1] and the cells between them fit the
- * wave). Output row r takes the corner fluxes at rows r and r+1, the second
- * from a window of u0 rows r-1 .. r+2; ghost faces hold the negated mirror
- * face's u1 (INVERT_X) and ghost cells the mirror cell's flux. */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
-    const int q = x0 - 3 + lane;
-    const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
-    const int qg = min(max(q, 1), nx);
-    const bool face_ghost = is_ghost(q, sx), cell_ghost = is_ghost(q, nx);
-    const int mf_lane = mirror_lane(lane, q, sx), mc_lane = mirror_lane(lane, q, nx);
-    const bool own = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
-    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-    const T iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
-      const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
-      return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
-                           v_cell, cdy(gf - 1), cdy(gf), dt_h);
-    };
-
-    /* u0 rows r-1 .. r+2 of face q (ghost rows are in memory) */
-    T wu[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) wu[k] = io.u[(y0 - 2 + k) * sx + qf];
-    T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], io.v0[y0 * nx + qc],
-                       io.F_y[y0 * nx + qc]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
-    T a_u = io.u[(y0 + 2) * sx + qf];
-    T a_v = io.v0[(y0 + 1) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
-    T a_m1 = io.mom1[y0 * sx + qf], a_r = io.rho[y0 * nx + qc];
-    T a_fx = io.F_x[y0 * sx + qf];
-    for (int r = y0; r < y1; ++r) {
-      wu[3] = a_u;
-      const T c_v = a_v, c_fy = a_fy, c_m1 = a_m1, c_r = a_r, c_fx = a_fx;
-      {
-        const int ru = min(r + 3, ny - 1), rv = min(r + 2, ny), rc = min(r + 1, ny - 1);
-        a_u = io.u[ru * sx + qf];
-        a_v = io.v0[rv * nx + qc];
-        a_fy = io.F_y[rv * nx + qc];
-        a_m1 = io.mom1[rc * sx + qf];
-        a_r = io.rho[rc * nx + qc];
-        a_fx = io.F_x[rc * sx + qf];
-      }
-      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
-      const T m1 = advect_update(c_m1, dt_h, uf, uf_prev, cdx_q, edy(r));
-      const T rex = rho_edge_x(c_r, up(c_r, 1), cdx_q, cdx_qm, cdy(r), edx_q);
-      T u1 = velocity(m1, rex);
-      u1 = mirror_fix(xedge, u1, mf_lane, face_ghost, T(-1.0));
-      const T up1 = dn(u1, 1);
-      T ufx = momentum_flux(edy(r), c_fx, dn(c_fx, 1), up(u1, 1), u1, up1, dn(u1, 2),
-                            iedx_q, T(0.5) * (u1 + up1), cdx_qm, cdx_q, dt_h);
-      ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
-      const T ufx_lo = up(ufx, 1);
-      if (own) {
-        io.mom[r * sx + q] = advect_update(m1, dt_h, ufx, ufx_lo, edx_q, cdy(r));
-        mput_xface(edge, io.vel, nx, ny, r, q, u1, T(-1.0));
-      }
-      uf_prev = uf;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
-    }
-  }
-}
-
-/* y momentum, odd tt (omp3/flow.c:621-669): y flux in x, v recompute, then
- * y flux in y. Lane l holds column q = x0 - 2 + l. Per y-face row k (two
- * rows ahead of the output row g) the corner fluxes along the row, the first
- * sub-update and v1; ghost y faces past the north wall are the negated
- * mirror (INVERT_Y). The cell fluxes at rows g-1 and g come from a window
- * of v1 rows g-1 .. g+2; a ghost cell row's flux is its mirror row's. io.u
- * is u after the x sub-updates. */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
-    const int q = x0 - 2 + lane;
-    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
-    const bool own = q >= x0 && q < x1;
-    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-    const T iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    /* v1 and m1 of interior y-face row k from row k's v0, mom1 and rows
-     * k, k-1 of F_x, u and rho */
-    auto yrow = [&](int k, T a_v, T a_m1, T a_fx, T p_fx, T a_u, T p_u, T a_r,
-                    T p_r, T& m1, T& v1) {
-      const T vm1 = up(a_v, 1);
-      const T vfx = momentum_flux(cdy(k), a_fx, p_fx, up(a_v, 2), vm1, a_v,
-                                  dn(a_v, 1), iedx_q, T(0.5) * (a_u + p_u),
-                                  cdx_qm, cdx_q, dt_h);
-      m1 = advect_update(a_m1, dt_h, dn(vfx, 1), vfx, edx_q, cdy(k));
-      const T rey = rho_edge_y(a_r, p_r, cdx_q, cdy(k), cdy(k - 1), edy(k));
-      v1 = velocity(m1, rey);
-    };
-    /* y flux in y at cell row i from v1 rows i-1 .. i+2 */
-    auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
-      return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
-                           T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
-    };
-
-    T wv[4], wm[3];
-    wm[0] = wm[1] = wm[2] = T(0);
-    /* rows y0-2 .. y0+1; below the south wall, the negated mirror rows */
-    T p_fx = T(0), p_u = T(0), p_r = T(0);
-    {
-      const int k0 = max(y0 - 2, kPad);
-      p_fx = io.F_x[(k0 - 1) * sx + qf];
-      p_u = io.u[(k0 - 1) * sx + qf];
-      p_r = io.rho[(k0 - 1) * nx + qc];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int k = y0 - 2 + s;
-        if (k < kPad) continue;
-        const T c_fx = io.F_x[k * sx + qf], c_u = io.u[k * sx + qf];
-        const T c_r = io.rho[k * nx + qc];
-        T m1, v1;
-        yrow(k, io.v0[k * nx + qc], io.mom1[k * nx + qc], c_fx, p_fx, c_u, p_u,
-             c_r, p_r, m1, v1);
-        wv[s] = v1;
-        if (s >= 2) wm[s - 2] = m1;
-        p_fx = c_fx;
-        p_u = c_u;
-        p_r = c_r;
-      }
-    }
-    const bool south = (y0 == kPad);
-    if (south) {
-      wv[0] = -wv[3];
-      wv[1] = -wv[2];
-    }
-    T p_fy = io.F_y[y0 * nx + qc];
-    T vfy_prev = south ? T(0)
-                       : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
-                                  io.F_y[(y0 - 1) * nx + qc], p_fy);
-#pragma unroll
-    for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
-    /* lookahead row y0+2 and the F_y row g+1 */
-    int kn = min(y0 + 2, ny - 1);
-    T a_v = io.v0[(y0 + 2) * nx + qc], a_m1 = io.mom1[(y0 + 2) * nx + qc];
-    T a_fx = io.F_x[kn * sx + qf], a_u = io.u[kn * sx + qf], a_r = io.rho[kn * nx + qc];
-    T a_fy = io.F_y[(y0 + 1) * nx + qc];
-    for (int gr = y0; gr < gend; ++gr) {
-      const int k = gr + 2;
-      const T c_v = a_v, c_m1 = a_m1, c_fx = a_fx, c_u = a_u, c_r = a_r, c_fy = a_fy;
-      {
-        const int ky = min(k + 1, ny), kx = min(k + 1, ny - 1);
-        a_v = io.v0[ky * nx + qc];
-        a_m1 = io.mom1[ky * nx + qc];
-        a_fx = io.F_x[kx * sx + qf];
-        a_u = io.u[kx * sx + qf];
-        a_r = io.rho[kx * nx + qc];
-        a_fy = io.F_y[min(gr + 2, ny) * nx + qc];
-      }
-      T m1 = T(0), v1;
-      if (k < ny - 1) {
-        yrow(k, c_v, c_m1, c_fx, p_fx, c_u, p_u, c_r, p_r, m1, v1);
-      } else {
-        /* ghost y face: mirror of face ny-1 is ny-2 (k-1), of ny is ny-3 (k-3) */
-        v1 = -((k == ny - 1) ? wv[2] : wv[0]);
-      }
-      wv[3] = v1;
-      wm[2] = m1;
-      T vfy = (gr >= ny - kPad)
-                  ? vfy_prev
-                  : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
-      if (south && gr == y0) vfy_prev = vfy;
-      if (own) {
-        io.mom[gr * nx + q] = advect_update(wm[0], dt_h, vfy, vfy_prev, cdx_q, edy(gr));
-        mput_yface(edge, io.vel, nx, ny, gr, q, wv[1], T(-1.0));
-      }
-      vfy_prev = vfy;
-      p_fx = c_fx;
-      p_u = c_u;
-      p_r = c_r;
-      p_fy = c_fy;
-#pragma unroll
-      for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
-      wm[0] = wm[1];
-      wm[1] = wm[2];
-    }
-  }
-}
-
-/* y momentum, even tt (omp3/flow.c:724-774): y flux in y, v recompute, then
- * y flux in x. Lane l holds column q = x0 - 2 + l. Output y-face row g takes
- * the cell fluxes at rows g-1 and g (the second from a window of v0 rows
- * g-1 .. g+2; a ghost cell row's flux is its mirror row's), then v1 along the
- * row (ghost columns: the mirror column, unsigned) and the corner fluxes
- * across the lanes. io.u is u after the x sub-updates. */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
-    const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
-    const int q = x0 - 2 + lane;
-    const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
-    const bool col_ghost = is_ghost(q, nx);
-    const int mc_lane = mirror_lane(lane, q, nx);
-    const bool own = q >= x0 && q < x1;
-    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-    const T iedx_q = g.iedx[qg];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-    const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
-
-    auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
-      return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
-                           T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
-    };
-
-    /* v0 rows g-1 .. g+2 (ghost rows are in memory) */
-    T wv[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) wv[s] = io.v0[(y0 - 2 + s) * nx + qc];
-    T p_fy = io.F_y[y0 * nx + qc];
-    const bool south = (y0 == kPad);
-    T vfy_prev = south ? T(0)
-                       : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
-                                  io.F_y[(y0 - 1) * nx + qc], p_fy);
-#pragma unroll
-    for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
-    T p_fx = io.F_x[(y0 - 1) * sx + qf], p_u = io.u[(y0 - 1) * sx + qf];
-    T p_r = io.rho[(y0 - 1) * nx + qc];
-    T a_v = io.v0[(y0 + 2) * nx + qc], a_fy = io.F_y[(y0 + 1) * nx + qc];
-    T a_fx = io.F_x[y0 * sx + qf], a_u = io.u[y0 * sx + qf];
-    T a_r = io.rho[y0 * nx + qc], a_m1 = io.mom1[y0 * nx + qc];
-    for (int gr = y0; gr < gend; ++gr) {
-      wv[3] = a_v;
-      const T c_fy = a_fy, c_fx = a_fx, c_u = a_u, c_r = a_r, c_m1 = a_m1;
-      {
-        const int rx = min(gr + 1, ny - 1);
-        a_v = io.v0[min(gr + 3, ny) * nx + qc];
-        a_fy = io.F_y[min(gr + 2, ny) * nx + qc];
-        a_fx = io.F_x[rx * sx + qf];
-        a_u = io.u[rx * sx + qf];
-        a_r = io.rho[rx * nx + qc];
-        a_m1 = io.mom1[min(gr + 1, ny) * nx + qc];
-      }
-      T vfy = (gr >= ny - kPad)
-                  ? vfy_prev
-                  : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
-      if (south && gr == y0) vfy_prev = vfy;
-      const T m1 = advect_update(c_m1, dt_h, vfy, vfy_prev, cdx_q, edy(gr));
-      const T rey = rho_edge_y(c_r, p_r, cdx_q, cdy(gr), cdy(gr - 1), edy(gr));
-      T v1 = velocity(m1, rey);
-      v1 = mirror_fix(xedge, v1, mc_lane, col_ghost, T(1.0));
-      const T vm1 = up(v1, 1);
-      const T vfx = momentum_flux(cdy(gr), c_fx, p_fx, up(v1, 2), vm1, v1, dn(v1, 1),
-                                  iedx_q, T(0.5) * (c_u + p_u), cdx_qm, cdx_q, dt_h);
-      const T vfx_hi = dn(vfx, 1);
-      if (own) {
-        io.mom[gr * nx + q] = advect_update(m1, dt_h, vfx_hi, vfx, edx_q, cdy(gr));
-        mput_yface(edge, io.vel, nx, ny, gr, q, v1, T(-1.0));
-      }
-      vfy_prev = vfy;
-      p_fy = c_fy;
-      p_fx = c_fx;
-      p_u = c_u;
-      p_r = c_r;
-#pragma unroll
-      for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
-    }
-  }
-}
-
-/* ======================================================================= */
-/* Lagrangian step                                                          */
-/* ======================================================================= */
-
-template <typename T>
-struct MLagIO {
-  const T* __restrict__ rho;
-  const T* __restrict__ e;
-  const T* __restrict__ mom_x;
-  const T* __restrict__ mom_y;
-  T* __restrict__ mx1;     /* x momentum after the viscous step */
-  T* __restrict__ my1;     /* y momentum after the viscous step */
-  T* __restrict__ u0;      /* x velocity after the viscous step, + ghosts */
-  T* __restrict__ v0;      /* y velocity after the viscous step, + ghosts */
-  T* __restrict__ e1;      /* energy after shock heating, + ghosts */
-  T* __restrict__ rho_old; /* copy of rho over every cell */
-};
-
-/* EOS, pressure acceleration, artificial viscosity, shock heating, the CFL
- * candidates and the copy of rho (omp3/flow.c:55-70, 110-290, 73-96,
- * 302-305). Lane l holds x face / cell column q = x0 - 3 + l (strips of 58
- * columns: shock heating on cells [x0, x1) needs u after the viscous step on
- * faces [x0, x1], hence Qxx on cells [x0-1, x1], u after pressure
- * acceleration on faces [x0-2, x1+2] and the pressure one cell further
- * west). Output row i
- * takes, along x, u after pressure acceleration on the row's faces, Qxx on
- * its cells and u after the viscous acceleration; along y, the lane's
- * column carries v after pressure acceleration on y faces i .. i+3 (face
- * i+3 computed this step), Qyy on cell rows i and i+1 and v after the
- * viscous acceleration on faces i and i+1. Ghost faces hold the negated
- * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
-template <typename T>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
-                                                           const double* dtp,
-                                                           FkReduce red, double max_dt,
-                                                           int first_step, MarchList ml) {
-  const int nx = g.nx, ny = g.ny, sx = nx + 1;
-  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
-  const int lane = threadIdx.x % kWave;
-  double local_min = max_dt;
-  int nan_count = 0;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
-       item += gridDim.x * kMWaves) {
-    const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-    const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-    const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-    const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                      (y1 + 3 > ny - kPad);
-    const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
-    const bool last_chunk = (y1 == ny - kPad);
-    const int q = x0 - 3 + lane;
-    const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
-    const int qg = min(max(q, 1), nx);
-    const bool face_ghost = is_ghost(q, sx), cell_ghost = is_ghost(q, nx);
-    const int mf_lane = mirror_lane(lane, q, sx), mc_lane = mirror_lane(lane, q, nx);
-    const bool own_face = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
-    const bool own = q >= x0 && q < x1;
-    const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-    const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-
-    /* y face k of the lane's column after pressure acceleration, from cell
-     * rows k (rk, ek) and k-1 (rs, es) and mom_y at face k */
-    auto ypa = [&](int k, T rk, T ek, T rs, T es, T my, T& mom, T& rey) {
-      mom = accelerate(my, dt, eos_pressure(rk, ek), eos_pressure(rs, es), edy(k));
-      rey = rho_edge_y(rk, rs, cdx_q, cdy(k), cdy(k - 1), edy(k));
-      return velocity(mom, rey);
-    };
-
-    /* ---- prologue: y faces y0-2 .. y0+2 and v after the viscous step at y0 */
-    T wr[4], we[4];          /* rho, e at cell rows i .. i+3 */
-    T vp[5], pm[5], pr[5];   /* y faces y0-2 .. y0+2: v, mom, rho edge */
-    T qy_prev, qy_m1;        /* Qyy at cell rows i and i-1 */
-    {
-      T rr[6], ee[6];        /* cell rows y0-3 .. y0+2 */
-#pragma unroll
-      for (int s = 0; s < 6; ++s) {
-        const int row = max(y0 - 3 + s, 0);
-        rr[s] = io.rho[row * nx + qc];
-        ee[s] = io.e[row * nx + qc];
-      }
-#pragma unroll
-      for (int s = 0; s < 5; ++s) {
-        const int f = y0 - 2 + s;
-        pm[s] = pr[s] = T(0);
-        if (f >= kPad && f < ny - 1)
-          vp[s] = ypa(f, rr[s + 1], ee[s + 1], rr[s], ee[s], io.mom_y[f * nx + qc],
-                      pm[s], pr[s]);
-        else if (f == ny - 1)
-          vp[s] = -vp[s - 1 > 0 ? s - 1 : 0];
-        else if (f == ny)
-          vp[s] = -vp[s - 3 > 0 ? s - 3 : 0];
-        else
-          vp[s] = T(0);
-      }
-      if (y0 == kPad) {
-        vp[0] = -vp[3];
-        vp[1] = -vp[2];
-      }
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        wr[s] = rr[s + 3];
-        we[s] = ee[s + 3];
-      }
-      /* Qyy at cell rows y0 and y0-1 (below the south wall: row y0's) */
-      qy_prev = viscous_stress(rr[3], vp[1], vp[2], vp[3], vp[4]);
-      qy_m1 = (y0 == kPad) ? qy_prev
-                           : viscous_stress(rr[2], vp[0], vp[1], vp[2], vp[3]);
-    }
-    T vav_prev;
-    {
-      const T mom = accelerate(pm[2], dt, qy_prev, qy_m1, cdy(y0));
-      vav_prev = velocity(mom, pr[2]);
-      if (own) {
-        io.my1[y0 * nx + q] = mom;
-        mput_yface(edge, io.v0, nx, ny, y0, q, vav_prev, T(-1.0));
-      }
-    }
-    /* windows entering row y0: v at faces y0 .. y0+2; mom, rho edge at faces
-     * y0+1, y0+2 */
-    T wv[4] = {vp[2], vp[3], vp[4], T(0)};
-    T wm[3] = {pm[3], pm[4], T(0)}, wx[3] = {pr[3], pr[4], T(0)};
-    /* row y0+3 and the row-y0 x inputs */
-    {
-      const int r3 = min(y0 + 3, ny - 1);
-      wr[3] = io.rho[r3 * nx + qc];
-      we[3] = io.e[r3 * nx + qc];
-    }
-    T a_my = io.mom_y[min(y0 + 3, ny) * nx + qc];
-    T a_mx = io.mom_x[y0 * sx + qf];
-    T a_r, a_e;
-    {
-      const int r4 = min(y0 + 4, ny - 1);
-      a_r = io.rho[r4 * nx + qc];
-      a_e = io.e[r4 * nx + qc];
-    }
-
-    for (int i = y0; i < y1; ++i) {
-      const T c_my = a_my, c_mx = a_mx;
-      const T n_r = a_r, n_e = a_e;
-      {
-        a_my = io.mom_y[min(i + 4, ny) * nx + qc];
-        a_mx = io.mom_x[min(i + 1, ny - 1) * sx + qf];
-        const int r5 = min(i + 5, ny - 1);
-        a_r = io.rho[r5 * nx + qc];
-        a_e = io.e[r5 * nx + qc];
-      }
-      /* y face i+3 after pressure acceleration */
-      const int k = i + 3;
-      T vk;
-      if (k < ny - 1) {
-        vk = ypa(k, wr[3], we[3], wr[2], we[2], c_my, wm[2], wx[2]);
-      } else {
-        vk = -((k == ny - 1) ? wv[2] : wv[0]);
-        wm[2] = wx[2] = T(0);
-      }
-      wv[3] = vk;
-      /* Qyy at cell row i+1 (a ghost row is its mirror, row i) */
-      const T qy = (i + 1 >= ny - kPad) ? qy_prev
-                                         : viscous_stress(wr[1], wv[0], wv[1], wv[2], wv[3]);
-      /* y face i+1 after the viscous acceleration */
-      const T my_av = accelerate(wm[0], dt, qy, qy_prev, cdy(i + 1));
-      const T vav = velocity(my_av, wx[0]);
-      if (own && (i + 1 < y1 || last_chunk)) {
-        io.my1[(i + 1) * nx + q] = my_av;
-        mput_yface(edge, io.v0, nx, ny, i + 1, q, vav, T(-1.0));
-      }
-
-      /* along x on row i */
-      const T ri = wr[0], ei = we[0];
-      const T P = eos_pressure(ri, ei);
-      const T mx_pa = accelerate(c_mx, dt, P, up(P, 1), edx_q);
-      const T rex = rho_edge_x(ri, up(ri, 1), cdx_q, cdx_qm, cdy(i), edx_q);
-      T u_pa = velocity(mx_pa, rex);
-      u_pa = mirror_fix(xedge, u_pa, mf_lane, face_ghost, T(-1.0));
-      T qx = viscous_stress(ri, up(u_pa, 1), u_pa, dn(u_pa, 1), dn(u_pa, 2));
-      qx = mirror_fix(xedge, qx, mc_lane, cell_ghost, T(1.0));
-      const T mx_av = accelerate(mx_pa, dt, qx, up(qx, 1), cdx_q);
-      const T uav = velocity(mx_av, rex);
-      if (own_face) {
-        io.mx1[i * sx + q] = mx_av;
-        mput_xface(edge, io.u0, nx, ny, i, q, uav, T(-1.0));
-      }
-
-      /* shock heating, CFL candidate and the copy of rho on row i */
-      const T uav_e = dn(uav, 1);
-      if (own) {
-        const int c = i * nx + q;
-        const T es = shock_heating(ei, P, ri, qx, qy_prev, uav, uav_e, vav_prev, vav,
-                                   cdx_q, cdy(i), dt_h);
-        mput_cell(edge, io.e1, nx, ny, i, q, es);
-        io.rho_old[c] = ri;
-        if (edge) {
-          const bool w = (q == kPad), ea = (q == nx - kPad - 1);
-          const bool s = (i == kPad), no = (i == ny - kPad - 1);
-          for (int d = 1; d <= kPad; ++d) {
-            if (w) io.rho_old[c - d] = io.rho[c - d];
-            if (ea) io.rho_old[c + d] = io.rho[c + d];
-            if (s) io.rho_old[c - d * nx] = io.rho[c - d * nx];
-            if (no) io.rho_old[c + d * nx] = io.rho[c + d * nx];
-            for (int d2 = 1; d2 <= kPad; ++d2) {
-              if (s && w) io.rho_old[c - d * nx - d2] = io.rho[c - d * nx - d2];
-              if (s && ea) io.rho_old[c - d * nx + d2] = io.rho[c - d * nx + d2];
-              if (no && w) io.rho_old[c + d * nx - d2] = io.rho[c + d * nx - d2];
-              if (no && ea) io.rho_old[c + d * nx + d2] = io.rho[c + d * nx + d2];
-            }
-          }
-        }
-        const double cand = cell_timestep(double(es), double(qx), double(qy_prev),
-                                          double(ri), double(cdx_q), double(cdy(i)));
-        if (cand == cand)
-          local_min = lesser(local_min, cand);
-        else
-          ++nan_count;
-      }
-
-      /* advance the windows */
-      qy_prev = qy;
-      vav_prev = vav;
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        wv[s] = wv[s + 1];
-        wr[s] = wr[s + 1];
-        we[s] = we[s + 1];
-      }
-      wr[3] = n_r;
-      we[3] = n_e;
-      wm[0] = wm[1];
-      wm[1] = wm[2];
-      wx[0] = wx[1];
-      wx[1] = wx[2];
-    }
-  }
-  finish_timestep<kMNT>(local_min, nan_count, red, max_dt, first_step);
-}
-
-template <typename K>
-int march_grid(K kernel, int& per_cu, int nitems) {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    hipDeviceProp_t p;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipGetDeviceProperties(&p, dev) != hipSuccess)
-      cus = 256;
-    else
-      cus = p.multiProcessorCount;
-  }
-  if (per_cu == 0 &&
-      (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMNT, 0) !=
-           hipSuccess ||
-       per_cu < 1))
-    per_cu = 1;
-  long gsz = (long)cus * per_cu;
-  const long need = (nitems + kMWaves - 1) / kMWaves;
-  if (gsz > need) gsz = need;
-  if (gsz > kMaxReduceBlocks) gsz = kMaxReduceBlocks;
-  return gsz < 1 ? 1 : (int)gsz;
-}
 
 /* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
 constexpr int kMassW = kWave - 4;

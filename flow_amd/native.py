@@ -80,6 +80,19 @@ def lib():
         "flowhip_f32_widen": (i, [vp, vp, sz]),
         "flowhip_f32_boundary": (i, [mp, vp, i, i, i]),
         "flowhip_run_f32": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp]),
+        "flowhip_ensemble_new": (vp, [i, ctypes.POINTER(i), ctypes.POINTER(i), _dp, _dp, _dp, _dp, i]),
+        "flowhip_ensemble_delete": (None, [vp]),
+        "flowhip_ensemble_size": (i, [vp]),
+        "flowhip_ensemble_local_size": (i, [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "flowhip_ensemble_array": (vp, [vp, i, i]),
+        "flowhip_ensemble_set_state": (i, [vp, i, vp, vp]),
+        "flowhip_ensemble_get_array": (i, [vp, i, i, vp]),
+        "flowhip_ensemble_set_dt": (i, [vp, i, d, d]),
+        "flowhip_ensemble_set_graphs": (i, [vp, i]),
+        "flowhip_ensemble_set_schedule": (i, [vp, i, i]),
+        "flowhip_ensemble_get_schedule": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "flowhip_ensemble_autotune": (i, [vp, i, i, _dp]),
+        "flowhip_ensemble_run": (i, [vp, i, i, i, vp, vp, vp, _dp]),
         "allocate_data": (sz, [ctypes.POINTER(vp), sz]),
         "deallocate_data": (None, [vp]),
         "handle_boundary_2d": (None, [i, i, mp, vp, i, i]),
@@ -318,6 +331,87 @@ class DeviceState32:
                                       _ptr(mass), _ptr(energy), ctypes.byref(wall))
         return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
                 energy[:steps], wall.value)
+
+
+class Ensemble:
+    """The batching form (flowhip_ensemble_*): independent meshes of uneven
+    sizes advanced together, FP64 or FP32 state."""
+
+    def __init__(self, sizes, width=1.0, max_dt=1.0e30, dt=1.0e-4, fp32=False):
+        n = len(sizes)
+        nx = (ctypes.c_int * n)(*[int(a) for a in sizes])
+        ny = (ctypes.c_int * n)(*[int(a) for a in sizes])
+        arr = lambda v: (ctypes.c_double * n)(*([float(v)] * n))
+        self.ptr = lib().flowhip_ensemble_new(n, nx, ny, arr(width), arr(width),
+                                              arr(max_dt), arr(dt), 1 if fp32 else 0)
+        if not self.ptr:
+            raise RuntimeError("flowhip_ensemble_new failed")
+        self.n = n
+        self.sizes = list(sizes)
+        self.fp32 = fp32
+
+    def close(self):
+        if self.ptr:
+            lib().flowhip_ensemble_delete(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def local_size(self, i):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().flowhip_ensemble_local_size(self.ptr, i, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def set_state(self, i, rho, e):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        nx, ny = self.local_size(i)
+        assert rho.size == e.size == (nx + 1) * (ny + 1)
+        if lib().flowhip_ensemble_set_state(self.ptr, i, _ptr(rho), _ptr(e)) != 0:
+            raise ValueError("bad instance")
+
+    def get(self, i, name):
+        nx, ny = self.local_size(i)
+        buf = np.empty((nx + 1) * (ny + 1))
+        lib().flowhip_ensemble_get_array(self.ptr, i, ARRAY_NAMES.index(name), _ptr(buf))
+        return buf
+
+    def get_all(self, i):
+        return {k: self.get(i, k) for k in ARRAY_NAMES}
+
+    def set_graphs(self, on):
+        lib().flowhip_ensemble_set_graphs(self.ptr, 1 if on else 0)
+
+    def set_schedule(self, chunk_rows, order):
+        if lib().flowhip_ensemble_set_schedule(self.ptr, chunk_rows, order) != 0:
+            raise ValueError("bad schedule")
+
+    def schedule(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().flowhip_ensemble_get_schedule(self.ptr, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def autotune(self, trial_steps=3, tt0=0):
+        ms = ctypes.c_double()
+        lib().flowhip_ensemble_autotune(self.ptr, trial_steps, tt0, ctypes.byref(ms))
+        return self.schedule(), ms.value
+
+    def run(self, niters, tt0=0, pre_timestep=True):
+        """Returns (dt [inst][niters+1][2], mass [inst][niters],
+        energy [inst][niters], wall_seconds)."""
+        dt = np.zeros(self.n * 2 * (niters + 1))
+        mass = np.zeros(max(self.n * niters, 1))
+        energy = np.zeros(max(self.n * niters, 1))
+        wall = ctypes.c_double()
+        lib().flowhip_ensemble_run(self.ptr, niters, tt0, 1 if pre_timestep else 0,
+                                   _ptr(dt), _ptr(mass), _ptr(energy), ctypes.byref(wall))
+        return (dt.reshape(self.n, niters + 1, 2),
+                mass[: self.n * niters].reshape(self.n, niters),
+                energy[: self.n * niters].reshape(self.n, niters), wall.value)
 
 
 def device_name():

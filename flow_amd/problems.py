@@ -136,6 +136,49 @@ def multi_blast(n, seed=20250101, side=0.05, grid=4, p_bg=0.1,
     return rho, e
 
 
+def ensemble_specs(count=256, seed=20250102, n_lo=64, n_hi=1024,
+                   split_lo=0.3, split_hi=0.7, ratio_lo=10.0, ratio_hi=1000.0):
+    """Instances of the batching workload: per instance (seeded splitmix64,
+    four draws each, in this order) a side n log-uniform in [n_lo, n_hi]
+    (rounded down), a split position U[split_lo, split_hi], a pressure ratio
+    log-uniform in [ratio_lo, ratio_hi] and a direction draw. The high side
+    is rho=1, p=0.1*ratio, the low side rho=0.125, p=0.1 (a Sod-type shock
+    tube); even instances split along x, odd ones along y unless the
+    direction draw is below 0.5, which makes the high-pressure region a
+    centred square blast of side 2*(split - 0.3) + 0.1 instead.
+    Returns [(n, kind, split, ratio)]."""
+    rng = SplitMix64(seed)
+    u = rng.doubles(4 * count).reshape(count, 4)
+    out = []
+    for k in range(count):
+        n = int(np.floor(np.exp(np.log(n_lo) + u[k, 0] * (np.log(n_hi) - np.log(n_lo)))))
+        n = max(n_lo, min(n_hi, n))
+        split = split_lo + u[k, 1] * (split_hi - split_lo)
+        ratio = np.exp(np.log(ratio_lo) + u[k, 2] * (np.log(ratio_hi) - np.log(ratio_lo)))
+        if u[k, 3] < 0.5:
+            kind = "blast"
+        else:
+            kind = "sod_x" if k % 2 == 0 else "sod_y"
+        out.append((n, kind, float(split), float(ratio)))
+    return out
+
+
+def ensemble_instance(n, kind, split, ratio):
+    """(rho, e) of one ensemble instance (see ensemble_specs)."""
+    e_hi = 0.1 * ratio / ((GAMMA - 1.0) * 1.0)
+    e_lo = 0.1 / ((GAMMA - 1.0) * 0.125)
+    if kind == "sod_x":
+        return boxes(n, n, [(0.125, e_lo, 0.0, 0.0, 1.0, 1.0),
+                            (1.0, e_hi, 0.0, 0.0, split, 1.0)])
+    if kind == "sod_y":
+        return boxes(n, n, [(0.125, e_lo, 0.0, 0.0, 1.0, 1.0),
+                            (1.0, e_hi, 0.0, 0.0, 1.0, split)])
+    side = 2.0 * (split - 0.3) + 0.1
+    lo = 0.5 - 0.5 * side
+    return boxes(n, n, [(1.0, 0.1 / (GAMMA - 1.0), 0.0, 0.0, 1.0, 1.0),
+                        (1.0, e_hi, lo, lo, side, side)])
+
+
 PROBLEMS = {
     "sod_x": sod_x,
     "sod_y": sod_y,

@@ -80,6 +80,49 @@ int flowhip_run_f32(Mesh* mesh, float** arrays, int niters, int tt0,
                     int pre_timestep, double* dt_pairs, double* mass,
                     double* energy, double* wall_seconds);
 
+/* ---- ensemble (batching) form ----
+ * ninst independent meshes of uneven sizes (global_nx[i] x global_ny[i]
+ * interior cells, arch.params width/height/max_dt and .params dt per
+ * instance), FP64 or FP32 state (fp32 = 1: the precision form's rules).
+ * Every stage is one launch for the whole ensemble: persistent kernels
+ * walk one work list of row-march items over all instances; each instance
+ * keeps its own dt pair and totals and computes bit for bit what it
+ * computes alone. Steps can be replayed from captured hipGraphs. */
+typedef struct FlowHipEnsemble FlowHipEnsemble;
+FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
+                                      const int* global_ny, const double* width,
+                                      const double* height, const double* max_dt,
+                                      const double* dt_param, int fp32);
+void flowhip_ensemble_delete(FlowHipEnsemble* ens);
+int flowhip_ensemble_size(FlowHipEnsemble* ens);
+/* Local sizes (pad included) of instance inst. */
+int flowhip_ensemble_local_size(FlowHipEnsemble* ens, int inst, int* nx, int* ny);
+/* Device pointer of array `which` (solve_hydro_2d order, 0..15) of an
+ * instance: (nx+1)*(ny+1) doubles, or floats in an FP32 ensemble. */
+void* flowhip_ensemble_array(FlowHipEnsemble* ens, int inst, int which);
+/* Initial density and energy (host FP64, (nx+1)*(ny+1) each) of one
+ * instance, reflected as main.c:56-59 does. */
+int flowhip_ensemble_set_state(FlowHipEnsemble* ens, int inst, const double* rho,
+                               const double* e);
+/* One array of one instance, widened to FP64, into host memory. */
+int flowhip_ensemble_get_array(FlowHipEnsemble* ens, int inst, int which,
+                               double* host);
+int flowhip_ensemble_set_dt(FlowHipEnsemble* ens, int inst, double dt, double dt_h);
+int flowhip_ensemble_set_graphs(FlowHipEnsemble* ens, int on);
+/* Chunk height (4..120) and work order (0: by instance, 1: largest
+ * instance first, 2: one chunk row of every instance in turn). */
+int flowhip_ensemble_set_schedule(FlowHipEnsemble* ens, int chunk_rows, int order);
+int flowhip_ensemble_get_schedule(FlowHipEnsemble* ens, int* chunk_rows, int* order);
+/* Times trial_steps steps of each chunk height and work order from the
+ * current state, keeps the fastest, restores the state. */
+int flowhip_ensemble_autotune(FlowHipEnsemble* ens, int trial_steps, int tt0,
+                              double* best_ms_per_step);
+/* niters steps of every instance; dt_pairs [inst][niters+1][2], mass and
+ * energy [inst][niters] (any may be NULL). Returns niters. */
+int flowhip_ensemble_run(FlowHipEnsemble* ens, int niters, int tt0,
+                         int pre_timestep, double* dt_pairs, double* mass,
+                         double* energy, double* wall_seconds);
+
 /* Device helpers for callers that manage their own buffers. */
 int flowhip_device_sync(void);
 int flowhip_memset_zero(double* dev, size_t len);

@@ -222,6 +222,50 @@ hipError_t fk_march_momentum_y_f32(FkMesh32 m, int tt, const float* u1,
                                    const float* my1, float* mom_y, float* v,
                                    const double* dtp, int chunk_rows,
                                    hipStream_t s);
+/* ---- ensemble (batching) form ----
+ * One descriptor per instance; arrays and geometry are FP64 or FP32 (one
+ * type for the whole ensemble). Work-list entries are (instance, item)
+ * pairs, items numbered strip-fastest within their instance. */
+typedef struct FkEnsInst {
+  int nx, ny;              /* local sizes, pad included */
+  const void* geo[6];      /* edgedx, edgedy, celldx, celldy, 1/edgedx, 1/edgedy */
+  void* a[16];             /* solve_hydro_2d's arrays, in its order */
+  double* dtp;             /* {dt, dt_h} of this instance (device) */
+  double max_dt;
+  int nstrips, nchunks, nitems;
+  int item0;               /* first partial slot of this instance */
+} FkEnsInst;
+
+typedef struct FkEnsPartials {
+  double* min;   /* per item: CFL minimum */
+  int* nan;      /* per item: undefined CFL candidates */
+  double* sums;  /* per item: compensated mass and energy (4 doubles) */
+} FkEnsPartials;
+
+typedef struct FkEnsHistory {
+  double* dt;    /* [step][instance][2]: dt pair after the step */
+  double* sums;  /* [step][instance][2]: mass and energy totals */
+  int* nan;      /* [step][instance]: undefined CFL candidates */
+  int* counter;  /* device step counter (the history row) */
+} FkEnsHistory;
+
+typedef struct FkEnsLaunch {
+  const FkEnsInst* insts;  /* device, ninst descriptors */
+  const int* work;         /* device, nwork (instance, item) pairs */
+  int ninst, nwork, chunk_rows;
+  FkEnsPartials part;
+  FkEnsHistory hist;
+} FkEnsLaunch;
+
+/* Strip width of the ensemble work items. */
+int fk_ens_strip_width(void);
+/* One step (omp3/flow.c:10-52) of every instance: Lagrangian stage, dt,
+ * mass/energy advection, totals, x and y momentum, counter advance. */
+hipError_t fk_ens_step(FkEnsLaunch L, int tt, int fp32, hipStream_t s);
+/* The pre-loop set_timestep (first_step = 0) of every instance. */
+hipError_t fk_ens_timestep(FkEnsLaunch L, int fp32, hipStream_t s);
+hipError_t fk_ens_set_counter(int* counter, int value, hipStream_t s);
+
 /* dst[k] = (float)src[k] / (double)src[k], k < n (round to nearest). */
 hipError_t fk_narrow(long n, float* dst, const double* src, hipStream_t s);
 hipError_t fk_widen(long n, double* dst, const float* src, hipStream_t s);
