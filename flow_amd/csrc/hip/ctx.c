@@ -69,7 +69,7 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   if (!h) return;
   hipStreamSynchronize(h->stream);
   for (int g = 0; g < 3; ++g)
-    if (h->graph[g]) hipGraphExecDestroy(h->graph[g]);
+    if (h->graph_ok[g]) hipGraphExecDestroy(h->graph[g]);
   if (h->events_created) {
     for (int k = 0; k < FLOWHIP_MAX_PENDING; ++k) {
       hipEventDestroy(h->ev_start[k]);

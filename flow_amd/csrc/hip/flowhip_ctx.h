@@ -30,6 +30,16 @@ typedef struct FlowHipTimer {
   long launches;
 } FlowHipTimer;
 
+/* One step's launch arguments (omp3/flow.c:10-52's arguments on the
+ * device side). */
+typedef struct FlowHipStep {
+  FkMesh m;
+  int tt;
+  double max_dt;
+  double *P, *rho, *rho_old, *e, *u, *v, *mom_x, *mom_y, *Qxx, *Qyy;
+  double *F_x, *F_y, *uF_x, *uF_y, *vF_x, *vF_y;
+} FlowHipStep;
+
 typedef struct FlowHipCtx {
   int device;
   hipStream_t stream;
@@ -47,8 +57,13 @@ typedef struct FlowHipCtx {
   int schedule;
   int march_rows;
   int use_graphs;
-  hipGraphExec_t graph[3]; /* even, odd, first step */
-  const void* graph_key[3][16];
+  /* captured steps: [0] even tt, [1] odd tt, [2] tt == 0; each valid for
+   * the launch arguments, schedule and chunk height it was captured with */
+  hipGraphExec_t graph[3];
+  int graph_ok[3];
+  FlowHipStep graph_step[3];
+  int graph_sched[3], graph_rows[3];
+  int graph_warm;
   /* conservation sums fused into the last advection kernel, valid while
    * the arrays they were taken from are unchanged */
   int sums_valid;

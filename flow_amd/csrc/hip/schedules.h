@@ -6,15 +6,8 @@
 
 #include "flowhip_ctx.h"
 
-typedef struct FlowHipStep {
-  FkMesh m;
-  int tt;
-  double max_dt;
-  double *P, *rho, *rho_old, *e, *u, *v, *mom_x, *mom_y, *Qxx, *Qyy;
-  double *F_x, *F_y, *uF_x, *uF_y, *vF_x, *vF_y;
-} FlowHipStep;
-
-/* Enqueue one step on h->stream (device dt pair already current). */
+/* Enqueue one step on h->stream (device dt pair already current); with
+ * graphs on, replayed from the step graph of its kind. */
 void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st);
 
 /* The two schedules. */

@@ -34,8 +34,10 @@ int flowhip_set_schedule(Mesh* mesh, int schedule);
 int flowhip_get_schedule(Mesh* mesh);
 /* Rows per chunk of the row-marching kernels: 4..120, default 32. */
 int flowhip_set_march_rows(Mesh* mesh, int rows);
-/* Replay each step from a captured hipGraph (one per sweep-order parity and
- * for the first step). */
+/* Replay each step from a captured hipGraph: one per sweep-order parity
+ * and one for tt == 0, captured on first use and recaptured when the
+ * arrays, geometry, schedule or chunk height change. Per-kernel timing
+ * (flowhip_set_timing) launches directly. */
 int flowhip_set_graphs(Mesh* mesh, int on);
 
 /* Per-kernel timing with hipEvents on the kernel set's stream. */
