@@ -2,6 +2,7 @@
  * ctx.c -- creation of the hip kernel set's per-Mesh state and its
  * per-kernel event timers.
  */
+#include <math.h>
 #include <string.h>
 
 #include "flowhip_ctx.h"
@@ -57,8 +58,12 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   memset(h->h_stage, 0, sizeof(double) * 8);
   memset(h->h_status, 0, sizeof(int) * 4);
   h->schedule = FLOWHIP_MARCH;
-  h->march_rows = 32;
+  h->march_rows = 48;
   schedule_from_env(h);
+  /* FLOWHIP_POW2=0 keeps IEEE quotients by spacings even on power-of-two
+   * meshes (the results are the same bits either way) */
+  const char* p2 = getenv("FLOWHIP_POW2");
+  h->no_pow2 = (p2 && !strcmp(p2, "0")) ? 1 : 0;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   return h;
@@ -91,12 +96,39 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   mesh->hip = NULL;
 }
 
-/* 1/edgedx and 1/edgedy, recomputed when the geometry arrays change. */
+/* Whether every value is a power of two with |exponent| <= emax. */
+static int all_pow2(const double* dev, int n, int emax) {
+  double* host = (double*)malloc(sizeof(double) * (size_t)n);
+  if (!host) TERMINATE("hip kernel set: out of host memory");
+  HIPCHECK(hipMemcpy(host, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  int ok = 1;
+  for (int k = 0; k < n && ok; ++k) {
+    int e = 0;
+    const double f = frexp(host[k], &e);
+    ok = (f == 0.5) && (e - 1 <= emax) && (e - 1 >= -emax);
+  }
+  free(host);
+  return ok;
+}
+
+/* 1/edgedx and 1/edgedy, recomputed when the geometry arrays change, and
+ * whether the spacings are powers of two. */
 static void ensure_reciprocals(FlowHipCtx* h, const Mesh* mesh) {
   const int lx = mesh->local_nx + 1, ly = mesh->local_ny + 1;
   if (h->inv_src_x == mesh->edgedx && h->inv_src_y == mesh->edgedy &&
+      h->inv_src_cx == mesh->celldx && h->inv_src_cy == mesh->celldy &&
       h->inv_nx == lx && h->inv_ny == ly)
     return;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  /* FP64: products of up to three spacings or reciprocals stay normal; the
+   * FP32 form's narrower range asks for smaller exponents */
+  h->pow2 = all_pow2(mesh->edgedx, lx, 300) && all_pow2(mesh->edgedy, ly, 300) &&
+            all_pow2(mesh->celldx, lx, 300) && all_pow2(mesh->celldy, ly, 300);
+  h->pow2_f32 = h->pow2 && all_pow2(mesh->edgedx, lx, 40) &&
+                all_pow2(mesh->edgedy, ly, 40) && all_pow2(mesh->celldx, lx, 40) &&
+                all_pow2(mesh->celldy, ly, 40);
+  h->inv_src_cx = mesh->celldx;
+  h->inv_src_cy = mesh->celldy;
   hipFree(h->inv_edgedx);
   hipFree(h->inv_edgedy);
   HIPCHECK(hipMalloc((void**)&h->inv_edgedx, sizeof(double) * lx));
@@ -116,8 +148,10 @@ FkMesh flowhip_fkmesh(const Mesh* mesh) {
     ensure_reciprocals(h, mesh);
     m.inv_edgedx = h->inv_edgedx;
     m.inv_edgedy = h->inv_edgedy;
+    m.pow2 = h->pow2 && !h->no_pow2;
   } else {
     m.inv_edgedx = m.inv_edgedy = NULL;
+    m.pow2 = 0;
   }
   m.nx = mesh->local_nx;
   m.ny = mesh->local_ny;
@@ -256,5 +290,6 @@ FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
   f.celldy = h->geo32 + 2 * lx + ly;
   f.inv_edgedx = h->geo32 + 2 * (lx + ly);
   f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
+  f.pow2 = h->pow2_f32 && !h->no_pow2;
   return f;
 }

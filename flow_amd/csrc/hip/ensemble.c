@@ -40,6 +40,7 @@ struct FlowHipEnsemble {
   int* d_work;
   int nwork;
   int chunk_rows, order;
+  int pow2; /* every instance's spacings are powers of two */
   FkEnsPartials part;
   int part_items;
   FkEnsHistory hist;
@@ -141,6 +142,7 @@ static FkEnsLaunch ens_launch(const FlowHipEnsemble* E) {
   L.ninst = E->ninst;
   L.nwork = E->nwork;
   L.chunk_rows = E->chunk_rows;
+  L.pow2 = E->pow2;
   L.part = E->part;
   L.hist = E->hist;
   return L;
@@ -240,6 +242,22 @@ FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
   }
   HIPCHECK(hipMemcpy(E->d_dtp, E->h_dtp, sizeof(double) * 2 * (size_t)ninst,
                      hipMemcpyHostToDevice));
+  /* uniform spacings: powers of two within the type's safe exponent range
+   * let the kernels divide by them as products with exact reciprocals */
+  E->pow2 = 1;
+  for (int i = 0; i < ninst && E->pow2; ++i) {
+    const double d[2] = {width[i] / global_nx[i], height[i] / global_ny[i]};
+    for (int k = 0; k < 2; ++k) {
+      int e = 0;
+      const double f = frexp(d[k], &e);
+      if (!(f == 0.5 && e - 1 <= (fp32 ? 40 : 300) && e - 1 >= -(fp32 ? 40 : 300)))
+        E->pow2 = 0;
+    }
+  }
+  {
+    const char* p2 = getenv("FLOWHIP_POW2");
+    if (p2 && !strcmp(p2, "0")) E->pow2 = 0;
+  }
   HIPCHECK(hipFree(dtmp));
   free(hbuf);
   free(aoff);

@@ -146,7 +146,8 @@ static int same_step(const FlowHipStep* a, const FlowHipStep* b) {
          a->m.ny == b->m.ny && a->m.pad == b->m.pad &&
          a->m.edgedx == b->m.edgedx && a->m.edgedy == b->m.edgedy &&
          a->m.celldx == b->m.celldx && a->m.celldy == b->m.celldy &&
-         a->m.inv_edgedx == b->m.inv_edgedx && a->m.inv_edgedy == b->m.inv_edgedy;
+         a->m.inv_edgedx == b->m.inv_edgedx && a->m.inv_edgedy == b->m.inv_edgedy &&
+         a->m.pow2 == b->m.pow2;
 }
 
 void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {

@@ -73,7 +73,7 @@ __device__ __forceinline__ int wave_sum_int(int x) {
 
 /* ---- the four stages ---- */
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     int H, FkEnsPartials part) {
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
     double local_min = in.max_dt;
     int nan_count = 0;
-    lagrangian_item(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane,
+    lagrangian_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane,
                     local_min, nan_count);
     local_min = wave_min(local_min);
     nan_count = wave_sum_int(nan_count);
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     int H, int tt, FkEnsPartials part) {
@@ -113,9 +113,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
     if (tt % 2 == 0)
-      mass_xy_item(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
+      mass_xy_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
     else
-      mass_yx_item(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
+      mass_yx_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
     ms = wave_csum(ms);
     es = wave_csum(es);
     if (lane == 0) {
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
 }
 
 /* ydir 0: x momentum (u0 = uF_x), 1: y momentum (io.u = the u just written) */
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     int H, int tt, int ydir) {
@@ -145,17 +145,17 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
                          arr<T>(in, kFy),  arr<T>(in, kRho), arr<T>(in, kQxx),
                          arr<T>(in, kMomX), arr<T>(in, kU)};
       if (tt % 2)
-        momx_xy_item(g, io, dt, dt_h, ml, wi.y, lane);
+        momx_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
       else
-        momx_yx_item(g, io, dt, dt_h, ml, wi.y, lane);
+        momx_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
     } else {
       const MMomIO<T> io{arr<T>(in, kU),   arr<T>(in, kUFy), arr<T>(in, kFx),
                          arr<T>(in, kFy),  arr<T>(in, kRho), arr<T>(in, kQyy),
                          arr<T>(in, kMomY), arr<T>(in, kV)};
       if (tt % 2)
-        momy_xy_item(g, io, dt, dt_h, ml, wi.y, lane);
+        momy_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
       else
-        momy_yx_item(g, io, dt, dt_h, ml, wi.y, lane);
+        momy_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
     }
   }
 }
@@ -268,27 +268,32 @@ __global__ void k_ens_set_counter(int* counter, int v) {
 
 /* ---- launch helpers ---- */
 
-template <typename T>
-hipError_t ens_step(const FkEnsLaunch& L, int tt, hipStream_t s) {
+template <typename T, bool P2>
+hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   static int pc_lag = 0, pc_mass = 0, pc_mom = 0;
   const int nw = L.nwork;
   if (nw < 1 || L.ninst < 1) return hipErrorInvalidValue;
   const int2* work = (const int2*)L.work;
-  hipLaunchKernelGGL(k_ens_lagrangian<T>, dim3(march_grid(k_ens_lagrangian<T>, pc_lag, nw)),
+  hipLaunchKernelGGL((k_ens_lagrangian<T, P2>), dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, tt == 0 ? 1 : 0, L.hist);
-  hipLaunchKernelGGL(k_ens_mass<T>, dim3(march_grid(k_ens_mass<T>, pc_mass, nw)),
+  hipLaunchKernelGGL((k_ens_mass<T, P2>), dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, L.hist);
-  const int gm = march_grid(k_ens_momentum<T>, pc_mom, nw);
-  hipLaunchKernelGGL(k_ens_momentum<T>, dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
+  const int gm = march_grid(k_ens_momentum<T, P2>, pc_mom, nw);
+  hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
                      L.chunk_rows, tt, 0);
-  hipLaunchKernelGGL(k_ens_momentum<T>, dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
+  hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
                      L.chunk_rows, tt, 1);
   hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(1), 0, s, L.hist.counter);
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t ens_step(const FkEnsLaunch& L, int tt, hipStream_t s) {
+  return L.pow2 ? ens_step_p<T, true>(L, tt, s) : ens_step_p<T, false>(L, tt, s);
 }
 
 template <typename T>

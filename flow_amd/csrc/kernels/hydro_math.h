@@ -34,6 +34,18 @@ __device__ __forceinline__ float quot<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
+/* x / d for a divisor d made of mesh spacings. When every spacing of the
+ * mesh is a power of two (P2), d is one too and rd = 1/d is exact, so x * rd
+ * is the IEEE quotient bit for bit (division by a power of two only
+ * rescales) at the cost of one multiply; otherwise the quotient itself. */
+template <bool P2, typename T>
+__device__ __forceinline__ T gquot(T x, T d, T rd) {
+  if constexpr (P2)
+    return x * rd;
+  else
+    return quot(x, d);
+}
+
 /* arch min(a, b) = a < b ? a : b */
 template <typename T>
 __device__ __forceinline__ T lesser(T a, T b) {
@@ -66,18 +78,20 @@ __device__ __forceinline__ float eos_pressure<float>(float rho, float e) {
 
 /* Edge-centred density on an x face between cells j-1 (w) and j (c):
  * omp3/flow.c:134-137. */
-template <typename T>
+template <typename T, bool P2 = false>
 __device__ __forceinline__ T rho_edge_x(T rho_c, T rho_w, T cdx_c, T cdx_w,
-                                        T cdy, T edx) {
-  return quot(rho_c * cdx_c * cdy + rho_w * cdx_w * cdy, T(2.0) * edx * cdy);
+                                        T cdy, T edx, T r_den = T(0)) {
+  return gquot<P2>(rho_c * cdx_c * cdy + rho_w * cdx_w * cdy, T(2.0) * edx * cdy,
+                   r_den);
 }
 
 /* Edge-centred density on a y face between cells i-1 (s) and i (c):
  * omp3/flow.c:138-141. */
-template <typename T>
+template <typename T, bool P2 = false>
 __device__ __forceinline__ T rho_edge_y(T rho_c, T rho_s, T cdx, T cdy_c,
-                                        T cdy_s, T edy) {
-  return quot(rho_c * cdx * cdy_c + rho_s * cdx * cdy_s, T(2.0) * cdx * edy);
+                                        T cdy_s, T edy, T r_den = T(0)) {
+  return gquot<P2>(rho_c * cdx * cdy_c + rho_s * cdx * cdy_s, T(2.0) * cdx * edy,
+                   r_den);
 }
 
 /* Velocity from momentum with the zero-density guard, omp3/flow.c:144-150. */
@@ -88,17 +102,17 @@ __device__ __forceinline__ T velocity(T mom, T rho_edge) {
 
 /* mom -= dt * (a - b) / d: the pressure (omp3/flow.c:126-131) and viscous
  * (:216-219) accelerations. */
-template <typename T>
-__device__ __forceinline__ T accelerate(T mom, T dt, T a, T b, T d) {
-  return mom - quot(dt * (a - b), d);
+template <typename T, bool P2 = false>
+__device__ __forceinline__ T accelerate(T mom, T dt, T a, T b, T d, T rd = T(0)) {
+  return mom - gquot<P2>(dt * (a - b), d, rd);
 }
 
 /* mom -= dt_h * (a - b) / (d1 * d2): the momentum advection updates
  * (omp3/flow.c:581-584, 614-617, 634-637, 664-667). */
-template <typename T>
+template <typename T, bool P2 = false>
 __device__ __forceinline__ T advect_update(T mom, T dt_h, T a, T b, T d1,
-                                           T d2) {
-  return mom - quot(dt_h * (a - b), d1 * d2);
+                                           T d2, T rd12 = T(0)) {
+  return mom - gquot<P2>(dt_h * (a - b), d1 * d2, rd12);
 }
 
 /* Artificial viscous stress from the four face velocities w0..w3 around a
@@ -115,12 +129,13 @@ __device__ __forceinline__ T viscous_stress(T rho, T w0, T w1, T w2, T w3) {
 
 /* Shock heating and work on one cell, omp3/flow.c:262-283. u0/u1 are the
  * west/east face velocities, v0/v1 the south/north ones. */
-template <typename T>
+template <typename T, bool P2 = false>
 __device__ __forceinline__ T shock_heating(T e, T P, T rho, T qxx, T qyy,
                                            T u0, T u1, T v0, T v1, T cdx,
-                                           T cdy, T dt_h) {
-  const T dvx = quot(u1 - u0, cdx);
-  const T dvy = quot(v1 - v0, cdy);
+                                           T cdy, T dt_h, T rcdx = T(0),
+                                           T rcdy = T(0)) {
+  const T dvx = gquot<P2>(u1 - u0, cdx, rcdx);
+  const T dvy = gquot<P2>(v1 - v0, cdy, rcdy);
   const T div_dt = (dvx + dvy) * dt_h;
   const T e_q = e - quot(dt_h * (qxx * dvx + qyy * dvy), rho);
   const T rho_c = quot(rho, T(1.0) + div_dt);
@@ -177,10 +192,10 @@ struct FaceFlux {
  *   et           transverse edge length (edgedy[i] in x, edgedx[j] in y)
  * The y sweep selects the Van Leer smoothness by the face velocity v0
  * instead of the time-centred one (omp3/flow.c:477 vs :361). */
-template <typename T, bool kYSweep>
+template <typename T, bool kYSweep, bool P2 = false>
 __device__ __forceinline__ FaceFlux<T> mass_energy_flux(
     T vm, T v0, T vp, T rmm, T rm, T rp, T rpp, T emm, T em, T ep, T epp,
-    T inv, T cd_lo, T cd_hi, T et, T dt, T dt_h) {
+    T inv, T cd_lo, T cd_hi, T et, T dt, T dt_h, T r_cd_hi = T(0)) {
   const T s_c = T(0.5) * inv * (vp - vm);
   const T s_r = T(2.0) * inv * (v0 - vm);
   const T s_l = T(2.0) * inv * (vp - v0);
@@ -197,7 +212,7 @@ __device__ __forceinline__ FaceFlux<T> mass_energy_flux(
   const T rho_up = (v_tc >= T(0.0)) ? rm : rp;
   FaceFlux<T> out;
   out.mass = (v_tc * rho_up + T(0.5) * vabs(v_tc) *
-                                  (T(1.0) - vabs(quot(v_tc * dt_h, cd_hi))) *
+                                  (T(1.0) - vabs(gquot<P2>(v_tc * dt_h, cd_hi, r_cd_hi))) *
                                   limiter * drho);
   const T e_face = mc_upwind(emm, em, ep, epp, inv, v_tc, v_tc > T(0.0),
                              cd_lo, cd_hi, dt_h);
@@ -213,13 +228,13 @@ struct CellState {
   T rho;
   T e;
 };
-template <typename T>
+template <typename T, bool P2 = false>
 __device__ __forceinline__ CellState<T> sweep_update(
     T rho, T rho_old, T e, T f_lo, T f_hi, T ef_lo, T ef_hi, T et_lo, T et_hi,
-    T cdx, T cdy, T dt_h, bool first) {
+    T cdx, T cdy, T dt_h, bool first, T r_cxy = T(0)) {
   CellState<T> out;
-  out.rho = rho - quot(dt_h * (et_hi * f_hi - et_lo * f_lo), cdx * cdy);
-  const T rho_e = (rho_old * e - quot(dt_h * (ef_hi - ef_lo), cdx * cdy));
+  out.rho = rho - gquot<P2>(dt_h * (et_hi * f_hi - et_lo * f_lo), cdx * cdy, r_cxy);
+  const T rho_e = (rho_old * e - gquot<P2>(dt_h * (ef_hi - ef_lo), cdx * cdy, r_cxy));
   out.e = first ? ((rho_old == T(0.0)) ? T(0.0) : quot(rho_e, rho_old))
                 : ((out.rho == T(0.0)) ? T(0.0) : quot(rho_e, out.rho));
   return out;

@@ -126,10 +126,16 @@ template <typename T>
 struct RowGeo {
   T a, b;
   int base;
-  __device__ RowGeo(const T* arr, int base_row, int last_row, int lane) : base(base_row) {
+  __device__ RowGeo(const T* arr, int base_row, int last_row, int lane,
+                    bool recip = false)
+      : base(base_row) {
     /* rows outside [0, last_row] are never asked for; clamp the loads */
     a = arr[max(0, min(base_row + lane, last_row))];
     b = arr[max(0, min(base_row + kWave + lane, last_row))];
+    if (recip) {
+      a = T(1.0) / a;
+      b = T(1.0) / b;
+    }
   }
   /* value at row r (uniform, base <= r < base + 128) */
   __device__ __forceinline__ T operator()(int r) const {
@@ -191,7 +197,7 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * [x0, x1] and the first (x) update of cells [x0, x1); per lane a window of
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -215,6 +221,9 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* rows of the x-updated fields for this lane's column: w[0] oldest */
   T wr[4], we[4];
@@ -250,15 +259,16 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
       const T um1 = up(a_u, 1), up1 = dn(a_u, 1);
       FaceFlux<T> fl{T(0), T(0)};
       if (face)
-        fl = mass_energy_flux<T, false>(um1, a_u, up1, rm2, rm1, a_r, rp1, em2, em1,
-                                        a_e, ep1, iedx_q, cdx_qm1, cdx_q,
-                                        edy(r), dt, dt_h);
+        fl = mass_energy_flux<T, false, P2>(um1, a_u, up1, rm2, rm1, a_r, rp1, em2,
+                                            em1, a_e, ep1, iedx_q, cdx_qm1, cdx_q,
+                                            edy(r), dt, dt_h, rcdx_q);
       if (own_face && r >= y0 && r < y1)
         mput_xface(edge, io.F_x, nx, ny, r, q, fl.mass, T(-1.0));
       const T f_hi = dn(fl.mass, 1), ef_hi = dn(fl.energy, 1);
       const CellState<T> st =
-          sweep_update(a_r, a_r, a_e, fl.mass, f_hi, fl.energy, ef_hi, edy(r),
-                       edy(r + 1), cdx_q, cdy(r), dt_h, true);
+          sweep_update<T, P2>(a_r, a_r, a_e, fl.mass, f_hi, fl.energy, ef_hi, edy(r),
+                              edy(r + 1), cdx_q, cdy(r), dt_h, true,
+                              rcdx_q * rcdy(r));
       xr = st.rho;
       xe = st.e;
     } else {
@@ -283,17 +293,18 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
     /* y flux at face r-1 from rows r-3 .. r */
     const int gf = r - 1;
     if (gf >= y0 && gf <= y1) {
-      const FaceFlux<T> fl = mass_energy_flux<T, true>(
+      const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
           vy[0], vy[1], vy[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2],
-          we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_q, dt, dt_h);
+          we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_q, dt, dt_h, rcdy(gf));
       if (own && (gf < y1 || y1 == ny - kPad))
         mput_yface(edge, io.F_y, nx, ny, gf, q, fl.mass, T(-1.0));
       /* y update of row r-2 */
       const int i = r - 2;
       if (i >= y0 && own) {
         const CellState<T> st =
-            sweep_update(wr[1], r0[0], we[1], fy_prev, fl.mass, efy_prev,
-                         fl.energy, edx_q, edx_q1, cdx_q, cdy(i), dt_h, false);
+            sweep_update<T, P2>(wr[1], r0[0], we[1], fy_prev, fl.mass, efy_prev,
+                                fl.energy, edx_q, edx_q1, cdx_q, cdy(i), dt_h, false,
+                                rcdx_q * rcdy(i));
         mput_cell(edge, io.rho, nx, ny, i, q, st.rho);
         mput_cell(edge, io.e, nx, ny, i, q, st.e);
         ms.add(double(st.rho));
@@ -313,7 +324,7 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    mass_xy_item(g, io, dt, dt_h, ml, item, lane, ms, es);
+    mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
   finish_sums<kMNT>(ms, es, red);
 }
@@ -336,7 +347,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * column (the same column inside the domain). Per row r: the y flux at face
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -361,6 +372,9 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q, rcdx_m = T(1.0) / cdx_m;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2 of column qm */
   T wr[4], we[4], wv[3];
@@ -372,9 +386,9 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
 #pragma unroll
   for (int k = 0; k < 3; ++k) wv[k] = io.v0[(y0 - 1 + k) * nx + qm];
   /* y flux at face y0 (rows y0-2 .. y0+1) */
-  FaceFlux<T> fprev = mass_energy_flux<T, true>(
+  FaceFlux<T> fprev = mass_energy_flux<T, true, P2>(
       wv[0], wv[1], wv[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2], we[3],
-      iedy(y0), cdy(y0 - 1), cdy(y0), edx_m, dt, dt_h);
+      iedy(y0), cdy(y0 - 1), cdy(y0), edx_m, dt, dt_h, rcdy(y0));
   if (own && q == qm) mput_yface(edge, io.F_y, nx, ny, y0, q, fprev.mass, T(-1.0));
   /* shift: windows now hold rows r-2 .. r+1 for r = y0; load row r+2 */
 #pragma unroll
@@ -405,14 +419,15 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
     }
     /* y flux at face r+1 (rows r-1 .. r+2) and the y update of row r */
     const int gf = r + 1;
-    const FaceFlux<T> fl = mass_energy_flux<T, true>(
+    const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
         wv[0], wv[1], wv[2], wr[0], wr[1], wr[2], wr[3], we[0], we[1], we[2],
-        we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_m, dt, dt_h);
+        we[3], iedy(gf), cdy(gf - 1), cdy(gf), edx_m, dt, dt_h, rcdy(gf));
     if (own && q == qm && (gf < y1 || y1 == ny - kPad))
       mput_yface(edge, io.F_y, nx, ny, gf, q, fl.mass, T(-1.0));
     const CellState<T> ys =
-        sweep_update(wr[1], wr[1], we[1], fprev.mass, fl.mass, fprev.energy,
-                     fl.energy, edx_m, edx_m1, cdx_m, cdy(r), dt_h, true);
+        sweep_update<T, P2>(wr[1], wr[1], we[1], fprev.mass, fl.mass, fprev.energy,
+                            fl.energy, edx_m, edx_m1, cdx_m, cdy(r), dt_h, true,
+                            rcdx_m * rcdy(r));
     fprev = fl;
     /* x fluxes at face q of the y-updated row, then the x update */
     const T rm2 = up(ys.rho, 2), rm1 = up(ys.rho, 1), rp1 = dn(ys.rho, 1);
@@ -420,15 +435,16 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
     const T um1 = up(c_u, 1), up1 = dn(c_u, 1);
     FaceFlux<T> fx{T(0), T(0)};
     if (face)
-      fx = mass_energy_flux<T, false>(um1, c_u, up1, rm2, rm1, ys.rho, rp1, em2, em1,
-                                      ys.e, ep1, iedx_q, cdx_qm1, cdx_q,
-                                      edy(r), dt, dt_h);
+      fx = mass_energy_flux<T, false, P2>(um1, c_u, up1, rm2, rm1, ys.rho, rp1, em2,
+                                          em1, ys.e, ep1, iedx_q, cdx_qm1, cdx_q,
+                                          edy(r), dt, dt_h, rcdx_q);
     if (own_face) mput_xface(edge, io.F_x, nx, ny, r, q, fx.mass, T(-1.0));
     const T f_hi = dn(fx.mass, 1), ef_hi = dn(fx.energy, 1);
     if (own) {
       const CellState<T> st =
-          sweep_update(ys.rho, c_r0, ys.e, fx.mass, f_hi, fx.energy, ef_hi,
-                       edy(r), edy(r + 1), cdx_q, cdy(r), dt_h, false);
+          sweep_update<T, P2>(ys.rho, c_r0, ys.e, fx.mass, f_hi, fx.energy, ef_hi,
+                              edy(r), edy(r + 1), cdx_q, cdy(r), dt_h, false,
+                              rcdx_q * rcdy(r));
       mput_cell(edge, io.rho, nx, ny, r, q, st.rho);
       mput_cell(edge, io.e, nx, ny, r, q, st.e);
       ms.add(double(st.rho));
@@ -444,7 +460,7 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -454,7 +470,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    mass_yx_item(g, io, dt, dt_h, ml, item, lane, ms, es);
+    mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
   finish_sums<kMNT>(ms, es, red);
 }
@@ -503,7 +519,7 @@ __device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
  * of the output row r, which takes the corner fluxes at rows r and r+1 from
  * a window of u1 rows r-1 .. r+2. Ghost rows are the mirror row's values
  * (u is copied unsigned across the south/north walls). */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -523,6 +539,9 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   const T iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* u1 and m1 of row i from that row's u0, F_x, mom1 and rho */
   auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
@@ -531,8 +550,10 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
                           dn(a_u, 2), iedx_q, T(0.5) * (a_u + up1), cdx_qm,
                           cdx_q, dt_h);
     ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
-    m1 = advect_update(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i));
-    const T rex = rho_edge_x(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q);
+    m1 = advect_update<T, P2>(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i),
+                              iedx_q * rcdy(i));
+    const T rex = rho_edge_x<T, P2>(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q,
+                                    T(0.5) * iedx_q * rcdy(i));
     u1 = velocity(m1, rex);
   };
   /* x flux in y at corner row gf from u1 rows gf-2 .. gf+1 */
@@ -577,7 +598,8 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
     wm[2] = m1;
     const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
     if (own) {
-      io.mom[r * sx + q] = advect_update(wm[0], dt_h, uf, uf_prev, cdx_q, edy(r));
+      io.mom[r * sx + q] = advect_update<T, P2>(wm[0], dt_h, uf, uf_prev, cdx_q, edy(r),
+                                                rcdx_q * iedy(r));
       mput_xface(edge, io.vel, nx, ny, r, q, wu[1], T(-1.0));
     }
     uf_prev = uf;
@@ -588,7 +610,7 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
@@ -596,7 +618,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGe
   const int lane = threadIdx.x % kWave;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    momx_xy_item(g, io, dt, dt_h, ml, item, lane);
+    momx_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
 
@@ -606,7 +628,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGe
  * wave). Output row r takes the corner fluxes at rows r and r+1, the second
  * from a window of u0 rows r-1 .. r+2; ghost faces hold the negated mirror
  * face's u1 (INVERT_X) and ghost cells the mirror cell's flux. */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -627,6 +649,9 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   const T iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
     const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
@@ -659,8 +684,10 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
       a_fx = io.F_x[rc * sx + qf];
     }
     const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], c_v, c_fy);
-    const T m1 = advect_update(c_m1, dt_h, uf, uf_prev, cdx_q, edy(r));
-    const T rex = rho_edge_x(c_r, up(c_r, 1), cdx_q, cdx_qm, cdy(r), edx_q);
+    const T m1 = advect_update<T, P2>(c_m1, dt_h, uf, uf_prev, cdx_q, edy(r),
+                                      rcdx_q * iedy(r));
+    const T rex = rho_edge_x<T, P2>(c_r, up(c_r, 1), cdx_q, cdx_qm, cdy(r), edx_q,
+                                    T(0.5) * iedx_q * rcdy(r));
     T u1 = velocity(m1, rex);
     u1 = mirror_fix(xedge, u1, mf_lane, face_ghost, T(-1.0));
     const T up1 = dn(u1, 1);
@@ -669,7 +696,8 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
     ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
     const T ufx_lo = up(ufx, 1);
     if (own) {
-      io.mom[r * sx + q] = advect_update(m1, dt_h, ufx, ufx_lo, edx_q, cdy(r));
+      io.mom[r * sx + q] = advect_update<T, P2>(m1, dt_h, ufx, ufx_lo, edx_q, cdy(r),
+                                                iedx_q * rcdy(r));
       mput_xface(edge, io.vel, nx, ny, r, q, u1, T(-1.0));
     }
     uf_prev = uf;
@@ -678,7 +706,7 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
@@ -686,7 +714,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGe
   const int lane = threadIdx.x % kWave;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    momx_yx_item(g, io, dt, dt_h, ml, item, lane);
+    momx_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
 
@@ -697,7 +725,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGe
  * mirror (INVERT_Y). The cell fluxes at rows g-1 and g come from a window
  * of v1 rows g-1 .. g+2; a ghost cell row's flux is its mirror row's. io.u
  * is u after the x sub-updates. */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -715,6 +743,9 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   const T iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* v1 and m1 of interior y-face row k from row k's v0, mom1 and rows
    * k, k-1 of F_x, u and rho */
@@ -724,8 +755,10 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
     const T vfx = momentum_flux(cdy(k), a_fx, p_fx, up(a_v, 2), vm1, a_v,
                                 dn(a_v, 1), iedx_q, T(0.5) * (a_u + p_u),
                                 cdx_qm, cdx_q, dt_h);
-    m1 = advect_update(a_m1, dt_h, dn(vfx, 1), vfx, edx_q, cdy(k));
-    const T rey = rho_edge_y(a_r, p_r, cdx_q, cdy(k), cdy(k - 1), edy(k));
+    m1 = advect_update<T, P2>(a_m1, dt_h, dn(vfx, 1), vfx, edx_q, cdy(k),
+                              iedx_q * rcdy(k));
+    const T rey = rho_edge_y<T, P2>(a_r, p_r, cdx_q, cdy(k), cdy(k - 1), edy(k),
+                                    T(0.5) * rcdx_q * iedy(k));
     v1 = velocity(m1, rey);
   };
   /* y flux in y at cell row i from v1 rows i-1 .. i+2 */
@@ -801,7 +834,8 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
                 : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
     if (south && gr == y0) vfy_prev = vfy;
     if (own) {
-      io.mom[gr * nx + q] = advect_update(wm[0], dt_h, vfy, vfy_prev, cdx_q, edy(gr));
+      io.mom[gr * nx + q] = advect_update<T, P2>(wm[0], dt_h, vfy, vfy_prev, cdx_q,
+                                                 edy(gr), rcdx_q * iedy(gr));
       mput_yface(edge, io.vel, nx, ny, gr, q, wv[1], T(-1.0));
     }
     vfy_prev = vfy;
@@ -816,7 +850,7 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
@@ -824,7 +858,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGe
   const int lane = threadIdx.x % kWave;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    momy_xy_item(g, io, dt, dt_h, ml, item, lane);
+    momy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
 
@@ -834,7 +868,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGe
  * g-1 .. g+2; a ghost cell row's flux is its mirror row's), then v1 along the
  * row (ghost columns: the mirror column, unsigned) and the corner fluxes
  * across the lanes. io.u is u after the x sub-updates. */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -855,6 +889,9 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   const T iedx_q = g.iedx[qg];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
     return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
@@ -893,8 +930,10 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
                 ? vfy_prev
                 : cellflux(gr, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
     if (south && gr == y0) vfy_prev = vfy;
-    const T m1 = advect_update(c_m1, dt_h, vfy, vfy_prev, cdx_q, edy(gr));
-    const T rey = rho_edge_y(c_r, p_r, cdx_q, cdy(gr), cdy(gr - 1), edy(gr));
+    const T m1 = advect_update<T, P2>(c_m1, dt_h, vfy, vfy_prev, cdx_q, edy(gr),
+                                      rcdx_q * iedy(gr));
+    const T rey = rho_edge_y<T, P2>(c_r, p_r, cdx_q, cdy(gr), cdy(gr - 1), edy(gr),
+                                    T(0.5) * rcdx_q * iedy(gr));
     T v1 = velocity(m1, rey);
     v1 = mirror_fix(xedge, v1, mc_lane, col_ghost, T(1.0));
     const T vm1 = up(v1, 1);
@@ -902,7 +941,8 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
                                 iedx_q, T(0.5) * (c_u + p_u), cdx_qm, cdx_q, dt_h);
     const T vfx_hi = dn(vfx, 1);
     if (own) {
-      io.mom[gr * nx + q] = advect_update(m1, dt_h, vfx_hi, vfx, edx_q, cdy(gr));
+      io.mom[gr * nx + q] = advect_update<T, P2>(m1, dt_h, vfx_hi, vfx, edx_q, cdy(gr),
+                                                 iedx_q * rcdy(gr));
       mput_yface(edge, io.vel, nx, ny, gr, q, v1, T(-1.0));
     }
     vfy_prev = vfy;
@@ -915,7 +955,7 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
                                                         const double* dtp,
                                                         MarchList ml) {
@@ -923,7 +963,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGe
   const int lane = threadIdx.x % kWave;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    momy_yx_item(g, io, dt, dt_h, ml, item, lane);
+    momy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
 
@@ -958,7 +998,7 @@ struct MLagIO {
  * i+3 computed this step), Qyy on cell rows i and i+1 and v after the
  * viscous acceleration on faces i and i+1. Ghost faces hold the negated
  * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, double& local_min, int& nan_count) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -979,12 +1019,17 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   const bool own = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
   const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T iedx_q = g.iedx[qg], rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane), rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* y face k of the lane's column after pressure acceleration, from cell
    * rows k (rk, ek) and k-1 (rs, es) and mom_y at face k */
   auto ypa = [&](int k, T rk, T ek, T rs, T es, T my, T& mom, T& rey) {
-    mom = accelerate(my, dt, eos_pressure(rk, ek), eos_pressure(rs, es), edy(k));
-    rey = rho_edge_y(rk, rs, cdx_q, cdy(k), cdy(k - 1), edy(k));
+    mom = accelerate<T, P2>(my, dt, eos_pressure(rk, ek), eos_pressure(rs, es), edy(k),
+                            iedy(k));
+    rey = rho_edge_y<T, P2>(rk, rs, cdx_q, cdy(k), cdy(k - 1), edy(k),
+                            T(0.5) * rcdx_q * iedy(k));
     return velocity(mom, rey);
   };
 
@@ -1030,7 +1075,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   }
   T vav_prev;
   {
-    const T mom = accelerate(pm[2], dt, qy_prev, qy_m1, cdy(y0));
+    const T mom = accelerate<T, P2>(pm[2], dt, qy_prev, qy_m1, cdy(y0), rcdy(y0));
     vav_prev = velocity(mom, pr[2]);
     if (own) {
       io.my1[y0 * nx + q] = mom;
@@ -1080,7 +1125,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     const T qy = (i + 1 >= ny - kPad) ? qy_prev
                                        : viscous_stress(wr[1], wv[0], wv[1], wv[2], wv[3]);
     /* y face i+1 after the viscous acceleration */
-    const T my_av = accelerate(wm[0], dt, qy, qy_prev, cdy(i + 1));
+    const T my_av = accelerate<T, P2>(wm[0], dt, qy, qy_prev, cdy(i + 1), rcdy(i + 1));
     const T vav = velocity(my_av, wx[0]);
     if (own && (i + 1 < y1 || last_chunk)) {
       io.my1[(i + 1) * nx + q] = my_av;
@@ -1090,13 +1135,14 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     /* along x on row i */
     const T ri = wr[0], ei = we[0];
     const T P = eos_pressure(ri, ei);
-    const T mx_pa = accelerate(c_mx, dt, P, up(P, 1), edx_q);
-    const T rex = rho_edge_x(ri, up(ri, 1), cdx_q, cdx_qm, cdy(i), edx_q);
+    const T mx_pa = accelerate<T, P2>(c_mx, dt, P, up(P, 1), edx_q, iedx_q);
+    const T rex = rho_edge_x<T, P2>(ri, up(ri, 1), cdx_q, cdx_qm, cdy(i), edx_q,
+                                    T(0.5) * iedx_q * rcdy(i));
     T u_pa = velocity(mx_pa, rex);
     u_pa = mirror_fix(xedge, u_pa, mf_lane, face_ghost, T(-1.0));
     T qx = viscous_stress(ri, up(u_pa, 1), u_pa, dn(u_pa, 1), dn(u_pa, 2));
     qx = mirror_fix(xedge, qx, mc_lane, cell_ghost, T(1.0));
-    const T mx_av = accelerate(mx_pa, dt, qx, up(qx, 1), cdx_q);
+    const T mx_av = accelerate<T, P2>(mx_pa, dt, qx, up(qx, 1), cdx_q, rcdx_q);
     const T uav = velocity(mx_av, rex);
     if (own_face) {
       io.mx1[i * sx + q] = mx_av;
@@ -1107,8 +1153,8 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     const T uav_e = dn(uav, 1);
     if (own) {
       const int c = i * nx + q;
-      const T es = shock_heating(ei, P, ri, qx, qy_prev, uav, uav_e, vav_prev, vav,
-                                 cdx_q, cdy(i), dt_h);
+      const T es = shock_heating<T, P2>(ei, P, ri, qx, qy_prev, uav, uav_e, vav_prev,
+                                        vav, cdx_q, cdy(i), dt_h, rcdx_q, rcdy(i));
       mput_cell(edge, io.e1, nx, ny, i, q, es);
       io.rho_old[c] = ri;
       if (edge) {
@@ -1153,7 +1199,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   }
 }
 
-template <typename T>
+template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
                                                            const double* dtp,
                                                            FkReduce red, double max_dt,
@@ -1164,7 +1210,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   int nan_count = 0;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    lagrangian_item(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+    lagrangian_item<T, P2>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
   finish_timestep<kMNT>(local_min, nan_count, red, max_dt, first_step);
 }

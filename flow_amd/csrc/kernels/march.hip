@@ -9,35 +9,35 @@ namespace flowhip {
 /* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
 constexpr int kMassW = kWave - 4;
 
-template <typename T>
-hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
-                             int chunk_rows, hipStream_t s) {
+template <typename T, bool P2>
+hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
+                               int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kMassW, chunk_rows);
   if (ml.nitems < 1) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_xy<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_mass_xy<T>, dim3(grid), dim3(kMNT), 0, s, g, io,
+    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_mass_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_yx<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_mass_yx<T>, dim3(grid), dim3(kMNT), 0, s, g, io,
+    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_mass_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   }
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
-                                   int first_step, int chunk_rows, hipStream_t s) {
+template <typename T, bool P2>
+hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
+                                     int first_step, int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
   if (ml.nitems < 1) return hipErrorInvalidValue;
   static int per_cu = 0;
-  const int grid = march_grid(k_march_lagrangian<T>, per_cu, ml.nitems);
-  hipLaunchKernelGGL(k_march_lagrangian<T>, dim3(grid), dim3(kMNT), 0, s, g, io,
+  const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems);
+  hipLaunchKernelGGL((k_march_lagrangian<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                      (const double*)r.dtp, r, max_dt, first_step, ml);
   return hipGetLastError();
 }
@@ -46,9 +46,9 @@ hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double m
 constexpr int kMomXW = kWave - 5;
 constexpr int kMomW = kWave - 4;
 
-template <typename T>
-hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
-                                 const double* dtp, int chunk_rows, hipStream_t s) {
+template <typename T, bool P2>
+hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
+                                   const double* dtp, int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const bool odd = (tt % 2) != 0;
   const MarchList ml =
@@ -56,22 +56,49 @@ hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
   if (ml.nitems < 1) return hipErrorInvalidValue;
   if (!ydir && odd) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momx_xy<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_momx_xy<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+    const int grid = march_grid(k_march_momx_xy<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momx_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
   } else if (!ydir) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momx_yx<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_momx_yx<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+    const int grid = march_grid(k_march_momx_yx<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momx_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
   } else if (odd) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momy_xy<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_momy_xy<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+    const int grid = march_grid(k_march_momy_xy<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momy_yx<T>, per_cu, ml.nitems);
-    hipLaunchKernelGGL(k_march_momy_yx<T>, dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
+    const int grid = march_grid(k_march_momy_yx<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io, dtp, ml);
   }
   return hipGetLastError();
+}
+
+/* Dispatch on whether every mesh spacing is a power of two (quotients by
+ * spacings become products with exact reciprocals). */
+template <typename T>
+hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
+                             int chunk_rows, int pow2, hipStream_t s) {
+  return pow2 ? launch_march_mass_p<T, true>(g, io, tt, r, chunk_rows, s)
+              : launch_march_mass_p<T, false>(g, io, tt, r, chunk_rows, s);
+}
+
+template <typename T>
+hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
+                                   int first_step, int chunk_rows, int pow2,
+                                   hipStream_t s) {
+  return pow2 ? launch_march_lagrangian_p<T, true>(g, io, r, max_dt, first_step,
+                                                   chunk_rows, s)
+              : launch_march_lagrangian_p<T, false>(g, io, r, max_dt, first_step,
+                                                    chunk_rows, s);
+}
+
+template <typename T>
+hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
+                                 const double* dtp, int chunk_rows, int pow2,
+                                 hipStream_t s) {
+  return pow2 ? launch_march_momentum_p<T, true>(g, io, tt, ydir, dtp, chunk_rows, s)
+              : launch_march_momentum_p<T, false>(g, io, tt, ydir, dtp, chunk_rows, s);
 }
 
 }  // namespace flowhip
@@ -93,7 +120,7 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
                                int first_step, int chunk_rows, hipStream_t s) {
   MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
   return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
-                                         chunk_rows, s);
+                                         chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
@@ -102,7 +129,7 @@ hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
                                const double* mx1, double* mom_x, double* u,
                                const double* dtp, int chunk_rows, hipStream_t s) {
   MMomIO<double> io{u0, v0, F_x, F_y, rho, mx1, mom_x, u};
-  return launch_march_momentum<double>(mgeo64(m), io, tt, 0, dtp, chunk_rows, s);
+  return launch_march_momentum<double>(mgeo64(m), io, tt, 0, dtp, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_y(FkMesh m, int tt, const double* u1,
@@ -111,7 +138,7 @@ hipError_t fk_march_momentum_y(FkMesh m, int tt, const double* u1,
                                const double* my1, double* mom_y, double* v,
                                const double* dtp, int chunk_rows, hipStream_t s) {
   MMomIO<double> io{u1, v0, F_x, F_y, rho, my1, mom_y, v};
-  return launch_march_momentum<double>(mgeo64(m), io, tt, 1, dtp, chunk_rows, s);
+  return launch_march_momentum<double>(mgeo64(m), io, tt, 1, dtp, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
@@ -120,7 +147,7 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s) {
   MMassIO<double> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, s);
+  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, s);
 }
 
 static MGeo<float> mgeo32(FkMesh32 m) {
@@ -137,7 +164,7 @@ hipError_t fk_march_lagrangian_f32(FkMesh32 m, const float* rho, const float* e,
                                    hipStream_t s) {
   MLagIO<float> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
   return launch_march_lagrangian<float>(mgeo32(m), io, r, max_dt, first_step,
-                                        chunk_rows, s);
+                                        chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
@@ -146,7 +173,7 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
                                     float* F_x, float* F_y, FkReduce r,
                                     int chunk_rows, hipStream_t s) {
   MMassIO<float> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, s);
+  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
@@ -156,7 +183,7 @@ hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
                                    const double* dtp, int chunk_rows,
                                    hipStream_t s) {
   MMomIO<float> io{u0, v0, F_x, F_y, rho, mx1, mom_x, u};
-  return launch_march_momentum<float>(mgeo32(m), io, tt, 0, dtp, chunk_rows, s);
+  return launch_march_momentum<float>(mgeo32(m), io, tt, 0, dtp, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_y_f32(FkMesh32 m, int tt, const float* u1,
@@ -166,7 +193,7 @@ hipError_t fk_march_momentum_y_f32(FkMesh32 m, int tt, const float* u1,
                                    const double* dtp, int chunk_rows,
                                    hipStream_t s) {
   MMomIO<float> io{u1, v0, F_x, F_y, rho, my1, mom_y, v};
-  return launch_march_momentum<float>(mgeo32(m), io, tt, 1, dtp, chunk_rows, s);
+  return launch_march_momentum<float>(mgeo32(m), io, tt, 1, dtp, chunk_rows, m.pow2, s);
 }
 
 }  // extern "C"

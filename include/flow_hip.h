@@ -32,7 +32,7 @@ extern "C" {
 
 int flowhip_set_schedule(Mesh* mesh, int schedule);
 int flowhip_get_schedule(Mesh* mesh);
-/* Rows per chunk of the row-marching kernels: 4..120, default 32. */
+/* Rows per chunk of the row-marching kernels: 4..120, default 48. */
 int flowhip_set_march_rows(Mesh* mesh, int rows);
 /* Replay each step from a captured hipGraph: one per sweep-order parity
  * and one for tt == 0, captured on first use and recaptured when the

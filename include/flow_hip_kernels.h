@@ -27,6 +27,9 @@ typedef struct FkMesh {
    * omp3/flow.c:344, 460), computed once per mesh by fk_reciprocal; read
    * by the fused kernels only. */
   const double *inv_edgedx, *inv_edgedy;
+  /* 1 when every spacing above is a power of two (the march kernels then
+   * divide by spacings as products with exact reciprocals), else 0 */
+  int pow2;
 } FkMesh;
 
 /* Geometry of the FP32-state precision form: the same arrays rounded to
@@ -35,6 +38,7 @@ typedef struct FkMesh32 {
   int nx, ny, pad;
   const float *edgedx, *edgedy, *celldx, *celldy;
   const float *inv_edgedx, *inv_edgedy;
+  int pow2;
 } FkMesh32;
 
 /* Device scratch for the single-pass grid reductions. */
@@ -253,6 +257,7 @@ typedef struct FkEnsLaunch {
   const FkEnsInst* insts;  /* device, ninst descriptors */
   const int* work;         /* device, nwork (instance, item) pairs */
   int ninst, nwork, chunk_rows;
+  int pow2;                /* every instance's spacings are powers of two */
   FkEnsPartials part;
   FkEnsHistory hist;
 } FkEnsLaunch;
