@@ -67,6 +67,8 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true",
                    help="skip the FP32-state precision form")
+    p.add_argument("--fp32-rows", type=int, default=None,
+                   help="chunk height of the FP32-state run (default: --march-rows)")
     p.add_argument("--no-ensemble", action="store_true",
                    help="skip the batching form")
     p.add_argument("--ens-count", type=int, default=256)
@@ -123,8 +125,9 @@ def fp32_leg(args, n, rho0, e0, ref_state, ref_dts, ref_mass, ref_energy,
     import flow_amd
     from flow_amd import native
     mesh = flow_amd.Mesh(n, n, dt=args.dt)
-    if args.march_rows:
-        mesh.set_march_rows(args.march_rows)
+    rows = args.fp32_rows or args.march_rows
+    if rows:
+        mesh.set_march_rows(rows)
     st = flow_amd.DeviceState32(mesh)
 
     def reset():

@@ -86,7 +86,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
                        arr<T>(in, kUFx),   arr<T>(in, kUFy), arr<T>(in, kP),
                        arr<T>(in, kRhoOld)};
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
-    double local_min = in.max_dt;
+    /* FP32 state: squared candidates (cell_timestep_sq) */
+    double local_min = std::is_same<T, float>::value ? in.max_dt * in.max_dt : in.max_dt;
     int nan_count = 0;
     lagrangian_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane,
                     local_min, nan_count);
@@ -206,18 +207,20 @@ __global__ __launch_bounds__(kMNT) void k_ens_timestep(
  * row *counter when hist != null. */
 __global__ __launch_bounds__(kWave) void k_ens_dt(const FkEnsInst* __restrict__ insts,
                                                   int ninst, FkEnsPartials part,
-                                                  int first_step, FkEnsHistory hist) {
+                                                  int first_step, int squared,
+                                                  FkEnsHistory hist) {
   const int inst = blockIdx.x;
   if (inst >= ninst) return;
   const FkEnsInst& in = insts[inst];
   const int lane = threadIdx.x;
-  double g = in.max_dt;
+  double g = squared ? in.max_dt * in.max_dt : in.max_dt;
   int bad = 0;
   for (int k = lane; k < in.nitems; k += kWave) {
     g = lesser(g, part.min[in.item0 + k]);
     bad += part.nan[in.item0 + k];
   }
   g = wave_min(g);
+  if (squared) g = __builtin_sqrt(g);
   bad = wave_sum_int(bad);
   if (lane == 0) {
     const double dt_h = in.dtp[1];
@@ -277,7 +280,8 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   hipLaunchKernelGGL((k_ens_lagrangian<T, P2>), dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
-                     L.part, tt == 0 ? 1 : 0, L.hist);
+                     L.part, tt == 0 ? 1 : 0, std::is_same<T, float>::value ? 1 : 0,
+                     L.hist);
   hipLaunchKernelGGL((k_ens_mass<T, P2>), dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
@@ -307,7 +311,7 @@ hipError_t ens_timestep(const FkEnsLaunch& L, hipStream_t s) {
                      dim3(kMNT), 0, s, L.insts, (const int2*)L.work, nw, L.chunk_rows,
                      L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
-                     L.part, 0, none);
+                     L.part, 0, 0, none);
   return hipGetLastError();
 }
 

@@ -144,6 +144,21 @@ __device__ __forceinline__ T shock_heating(T e, T P, T rho, T qxx, T qyy,
   return (rho == T(0.0)) ? T(0.0) : e_q - work;
 }
 
+/* The FP32-state form's CFL candidate, still in FP64: the square of
+ * omp3's min(dtx, dty), cdx^2 rho / (c^2 rho + 2 Qxx) with c^2 =
+ * gamma (gamma-1) e, so that no square root is taken per cell (the
+ * reduction takes one of the minimum). Cells with e < 0, where omp3's
+ * sound speed is undefined, give NaN as there. */
+__device__ __forceinline__ double cell_timestep_sq(double e, double qxx,
+                                                   double qyy, double rho,
+                                                   double cdx, double cdy) {
+  const double c2 = kGam * (kGam - 1.0) * e;
+  if (!(c2 >= 0.0)) return __builtin_nan("");
+  const double dx2 = (cdx * cdx * rho) / (c2 * rho + 2.0 * qxx);
+  const double dy2 = (cdy * cdy * rho) / (c2 * rho + 2.0 * qyy);
+  return lesser(dx2, dy2);
+}
+
 /* CFL candidate of one cell, omp3/flow.c:87-94. Always FP64. */
 __device__ __forceinline__ double cell_timestep(double e, double qxx,
                                                 double qyy, double rho,

@@ -1173,8 +1173,13 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
           }
         }
       }
-      const double cand = cell_timestep(double(es), double(qx), double(qy_prev),
-                                        double(ri), double(cdx_q), double(cdy(i)));
+      /* FP64 CFL candidate (squared in the FP32-state form) */
+      const double cand =
+          std::is_same<T, float>::value
+              ? cell_timestep_sq(double(es), double(qx), double(qy_prev), double(ri),
+                                 double(cdx_q), double(cdy(i)))
+              : cell_timestep(double(es), double(qx), double(qy_prev), double(ri),
+                              double(cdx_q), double(cdy(i)));
       if (cand == cand)
         local_min = lesser(local_min, cand);
       else
@@ -1206,13 +1211,14 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
                                                            int first_step, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  double local_min = max_dt;
+  constexpr bool kSq = std::is_same<T, float>::value;
+  double local_min = kSq ? max_dt * max_dt : max_dt;
   int nan_count = 0;
   for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
        item += gridDim.x * kMWaves) {
     lagrangian_item<T, P2>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
-  finish_timestep<kMNT>(local_min, nan_count, red, max_dt, first_step);
+  finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);
 }
 
 template <typename K>

@@ -65,17 +65,20 @@ __device__ __forceinline__ CSum block_csum(CSum v) {
 
 /* Per-block CFL minimum -> last block -> (dt, dt_h) update of
  * omp3/flow.c:103-105 and the undefined-candidate count. */
-template <int NT>
+/* kSquared: the candidates are squared time steps (the FP32-state form);
+ * the minimum's square root is taken once, here. */
+template <int NT, bool kSquared = false>
 __device__ void finish_timestep(double local_min, int nan_count, FkReduce r,
                                 double max_dt, int first_step) {
   if (nan_count) atomicAdd(r.status, nan_count);
   const double bmin = block_min<NT>(local_min);
   if (threadIdx.x == 0) r.partials[blockIdx.x] = bmin;
   if (!last_block_arrives(r.ticket)) return;
-  double g = max_dt;
+  double g = kSquared ? max_dt * max_dt : max_dt;
   for (int b = threadIdx.x; b < (int)gridDim.x; b += NT)
     g = lesser(g, r.partials[b]);
   g = block_min<NT>(g);
+  if (kSquared) g = __builtin_sqrt(g);
   if (threadIdx.x == 0) {
     const double dt_h = r.dtp[1];
     const double final_min = lesser(g, kCM * dt_h);
