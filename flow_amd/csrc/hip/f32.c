@@ -99,12 +99,9 @@ static void step_f32(FlowHipCtx* h, Mesh* mesh, FkMesh32 m, int tt, float** a) {
   FK_LAUNCH(h, "advect_mass_energy_f32",
             fk_march_mass_energy_f32(m, tt, uF_x, uF_y, rho_old, P, rho, e,
                                      F_x, F_y, h->red, rows, s));
-  FK_LAUNCH(h, "advect_momentum_x_f32",
-            fk_march_momentum_x_f32(m, tt, uF_x, uF_y, F_x, F_y, rho, Qxx,
-                                    mom_x, u, h->red.dtp, rows, s));
-  FK_LAUNCH(h, "advect_momentum_y_f32",
-            fk_march_momentum_y_f32(m, tt, u, uF_y, F_x, F_y, rho, Qyy, mom_y,
-                                    v, h->red.dtp, rows, s));
+  FK_LAUNCH(h, "advect_momentum_f32",
+            fk_march_momentum_f32(m, tt, uF_x, uF_y, F_x, F_y, rho, Qxx, Qyy,
+                                  mom_x, mom_y, u, v, h->red.dtp, rows, s));
 }
 
 static double now_seconds(void) {

@@ -39,14 +39,10 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
               fk_advect_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho,
                                     st->e, st->F_x, st->F_y, h->red, s));
   if (h->schedule == FLOWHIP_MARCH) {
-    FK_LAUNCH(h, "advect_momentum_x",
-              fk_march_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,
-                                  mx1, st->mom_x, st->u, h->red.dtp,
-                                  h->march_rows, s));
-    FK_LAUNCH(h, "advect_momentum_y",
-              fk_march_momentum_y(m, st->tt, st->u, v0, st->F_x, st->F_y,
-                                  st->rho, my1, st->mom_y, st->v, h->red.dtp,
-                                  h->march_rows, s));
+    FK_LAUNCH(h, "advect_momentum",
+              fk_march_momentum(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho, mx1,
+                                my1, st->mom_x, st->mom_y, st->u, st->v,
+                                h->red.dtp, h->march_rows, s));
   } else {
     FK_LAUNCH(h, "advect_momentum_x",
               fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,

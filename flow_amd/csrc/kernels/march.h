@@ -1221,6 +1221,397 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);
 }
 
+/* ======================================================================= */
+/* momentum, both directions in one pass                                    */
+/* ======================================================================= */
+
+template <typename T>
+struct MMom2IO {
+  const T* __restrict__ u0;   /* x velocity after the viscous step */
+  const T* __restrict__ v0;   /* y velocity after the viscous step */
+  const T* __restrict__ F_x;
+  const T* __restrict__ F_y;
+  const T* __restrict__ rho;  /* advected density */
+  const T* __restrict__ mx1;  /* momenta after the viscous step */
+  const T* __restrict__ my1;
+  T* __restrict__ mom_x;      /* advected momenta */
+  T* __restrict__ mom_y;
+  T* __restrict__ u;          /* recomputed velocities, + ghosts */
+  T* __restrict__ v;
+};
+
+/* Odd tt (omp3/flow.c:570-669): x momentum (x flux in x, u recompute, x flux
+ * in y) and y momentum (y flux in x, v recompute, y flux in y) of the same
+ * strip in one march. The y stage reads u after the x stage, which this
+ * wave has just computed for its lanes: row k's u, two rows ahead of the
+ * output row r, feeds both the x stage's corner window and the y stage's
+ * corner fluxes. Lane l holds x face / column q = x0 - 2 + l (strips of 59
+ * columns: the y stage's corner flux at face x1 needs u at x1, hence the x
+ * stage's cell flux at x1 and u0 up to x1+2). The per-row arithmetic is the
+ * two separate kernels', so the results are theirs bit for bit. */
+template <typename T, bool P2>
+__device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>& io,
+                                              T dt, T dt_h, const MarchList& ml,
+                                              int item, int lane) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  (void)dt;
+  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                    (y1 + 3 > ny - kPad);
+  const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+  const bool last_chunk = (y1 == ny - kPad);
+  const int gend = last_chunk ? y1 + 1 : y1;
+  const int q = x0 - 2 + lane;
+  const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
+  const bool cell_ghost = is_ghost(q, nx);
+  const int mc_lane = mirror_lane(lane, q, nx);
+  const bool own_x = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
+  const bool own_y = q >= x0 && q < x1;
+  const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+  const T iedx_q = g.iedx[qg];
+  const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+
+  /* x stage: u1 and m1 of cell row i from that row's u0, F_x, mx1, rho */
+  auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
+    const T up1 = dn(a_u, 1);
+    T ufx = momentum_flux(edy(i), a_fx, dn(a_fx, 1), up(a_u, 1), a_u, up1,
+                          dn(a_u, 2), iedx_q, T(0.5) * (a_u + up1), cdx_qm,
+                          cdx_q, dt_h);
+    ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
+    m1 = advect_update<T, P2>(a_m1, dt_h, ufx, up(ufx, 1), edx_q, cdy(i),
+                              iedx_q * rcdy(i));
+    const T rex = rho_edge_x<T, P2>(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q,
+                                    T(0.5) * iedx_q * rcdy(i));
+    u1 = velocity(m1, rex);
+  };
+  /* x stage: x flux in y at corner row gf from u1 rows gf-2 .. gf+1 */
+  auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
+    const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
+    return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
+                         v_cell, cdy(gf - 1), cdy(gf), dt_h);
+  };
+  /* y stage: v1 and m1 of y-face row k from v0, my1 at k and F_x, u, rho
+   * at rows k, k-1 */
+  auto yrow = [&](int k, T a_v, T a_m1, T a_fx, T p_fx, T a_u, T p_u, T a_r,
+                  T p_r, T& m1, T& v1) {
+    const T vm1 = up(a_v, 1);
+    const T vfx = momentum_flux(cdy(k), a_fx, p_fx, up(a_v, 2), vm1, a_v,
+                                dn(a_v, 1), iedx_q, T(0.5) * (a_u + p_u),
+                                cdx_qm, cdx_q, dt_h);
+    m1 = advect_update<T, P2>(a_m1, dt_h, dn(vfx, 1), vfx, edx_q, cdy(k),
+                              iedx_q * rcdy(k));
+    const T rey = rho_edge_y<T, P2>(a_r, p_r, cdx_q, cdy(k), cdy(k - 1), edy(k),
+                                    T(0.5) * rcdx_q * iedy(k));
+    v1 = velocity(m1, rey);
+  };
+  /* y stage: y flux in y at cell row i from v1 rows i-1 .. i+2 */
+  auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
+    return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
+                         T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
+  };
+
+  /* prologue. x stage on cell rows y0-3 .. y0+1 (a ghost row is its mirror
+   * row, as the boundary copies it); F_x and rho of those rows as the y
+   * stage reads them (their ghost rows are mirror copies too) */
+  T uu[5], ff[5], rr[5], wm[3];
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    const int i = mirror(y0 - 3 + s, ny);
+    const T a_fx = io.F_x[i * sx + qf], a_r = io.rho[i * nx + qc];
+    T m1, u1;
+    xrow(i, io.u0[i * sx + qf], a_fx, io.mx1[i * sx + qf], a_r, m1, u1);
+    uu[s] = u1;
+    ff[s] = a_fx;
+    rr[s] = a_r;
+    if (s >= 3) wm[s - 3] = m1;
+  }
+  T wu[4] = {uu[1], uu[2], uu[3], uu[4]};
+  T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], io.v0[y0 * nx + qc],
+                     io.F_y[y0 * nx + qc]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) wu[k] = wu[k + 1];
+  /* y stage on y faces y0-2 .. y0+1; below the south wall the negated
+   * mirror faces */
+  T wv[4], wn[3];
+  wn[0] = wn[1] = wn[2] = T(0);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = y0 - 2 + s;
+    wv[s] = T(0);
+    if (k < kPad) continue;
+    T m1, v1;
+    yrow(k, io.v0[k * nx + qc], io.my1[k * nx + qc], ff[s + 1], ff[s], uu[s + 1],
+         uu[s], rr[s + 1], rr[s], m1, v1);
+    wv[s] = v1;
+    if (s >= 2) wn[s - 2] = m1;
+  }
+  const bool south = (y0 == kPad);
+  if (south) {
+    wv[0] = -wv[3];
+    wv[1] = -wv[2];
+  }
+  T p_fy = io.F_y[y0 * nx + qc];
+  T vfy_prev = south ? T(0)
+                     : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
+                                io.F_y[(y0 - 1) * nx + qc], p_fy);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) wv[s] = wv[s + 1];
+  T p_fx = ff[4], p_u = uu[4], p_r = rr[4];
+  T p_v = io.v0[(y0 + 1) * nx + qc];
+  /* inputs of the lookahead row y0+2 */
+  int ia = mirror(y0 + 2, ny);
+  T a_u = io.u0[ia * sx + qf], a_fx = io.F_x[ia * sx + qf];
+  T a_mx = io.mx1[ia * sx + qf], a_r = io.rho[ia * nx + qc];
+  T a_v = io.v0[min(y0 + 2, ny) * nx + qc], a_my = io.my1[min(y0 + 2, ny) * nx + qc];
+  T a_fy = io.F_y[(y0 + 1) * nx + qc];
+  for (int r = y0; r < gend; ++r) {
+    const T c_u = a_u, c_fx = a_fx, c_mx = a_mx, c_r = a_r, c_v = a_v, c_my = a_my,
+            c_fy = a_fy;
+    {
+      const int in = mirror(r + 3, ny), ry = min(r + 3, ny);
+      a_u = io.u0[in * sx + qf];
+      a_fx = io.F_x[in * sx + qf];
+      a_mx = io.mx1[in * sx + qf];
+      a_r = io.rho[in * nx + qc];
+      a_v = io.v0[ry * nx + qc];
+      a_my = io.my1[ry * nx + qc];
+      a_fy = io.F_y[min(r + 2, ny) * nx + qc];
+    }
+    const int k = r + 2;
+    /* x stage on lookahead row k: u there feeds both stages */
+    T m1x, u1;
+    xrow(mirror(k, ny), c_u, c_fx, c_mx, c_r, m1x, u1);
+    wu[3] = u1;
+    wm[2] = m1x;
+    /* y stage on lookahead face k */
+    T m1y = T(0), v1;
+    if (k < ny - 1)
+      yrow(k, c_v, c_my, c_fx, p_fx, u1, p_u, c_r, p_r, m1y, v1);
+    else
+      /* ghost y face: mirror of face ny-1 is ny-2 (k-1), of ny is ny-3 (k-3) */
+      v1 = -((k == ny - 1) ? wv[2] : wv[0]);
+    wv[3] = v1;
+    wn[2] = m1y;
+    /* x stage output: cell row r */
+    if (r < y1) {
+      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], p_v, c_fy);
+      if (own_x) {
+        io.mom_x[r * sx + q] = advect_update<T, P2>(wm[0], dt_h, uf, uf_prev, cdx_q,
+                                                    edy(r), rcdx_q * iedy(r));
+        mput_xface(edge, io.u, nx, ny, r, q, wu[1], T(-1.0));
+      }
+      uf_prev = uf;
+    }
+    /* y stage output: y-face row r */
+    const T vfy = (r >= ny - kPad)
+                      ? vfy_prev
+                      : cellflux(r, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
+    if (south && r == y0) vfy_prev = vfy;
+    if (own_y) {
+      io.mom_y[r * nx + q] = advect_update<T, P2>(wn[0], dt_h, vfy, vfy_prev, cdx_q,
+                                                  edy(r), rcdx_q * iedy(r));
+      mput_yface(edge, io.v, nx, ny, r, q, wv[1], T(-1.0));
+    }
+    vfy_prev = vfy;
+    p_fx = c_fx;
+    p_u = u1;
+    p_r = c_r;
+    p_fy = c_fy;
+    p_v = c_v;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      wu[s] = wu[s + 1];
+      wv[s] = wv[s + 1];
+    }
+    wm[0] = wm[1];
+    wm[1] = wm[2];
+    wn[0] = wn[1];
+    wn[1] = wn[2];
+  }
+}
+
+/* Even tt (omp3/flow.c:673-774): x momentum (x flux in y, u recompute, x
+ * flux in x) and y momentum (y flux in y, v recompute, y flux in x) in one
+ * march; the y stage at row r reads u of rows r and r-1 from the x stage.
+ * Lane l holds x face / column q = x0 - 3 + l (strips of 59 columns). */
+template <typename T, bool P2>
+__device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>& io,
+                                              T dt, T dt_h, const MarchList& ml,
+                                              int item, int lane) {
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  (void)dt;
+  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
+  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                    (y1 + 3 > ny - kPad);
+  const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+  const bool last_chunk = (y1 == ny - kPad);
+  const int gend = last_chunk ? y1 + 1 : y1;
+  const int q = x0 - 3 + lane;
+  const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
+  const int qg = min(max(q, 1), nx);
+  const bool face_ghost = is_ghost(q, sx), cell_ghost = is_ghost(q, nx);
+  const int mf_lane = mirror_lane(lane, q, sx), mc_lane = mirror_lane(lane, q, nx);
+  const bool own_x = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
+  const bool own_y = q >= x0 && q < x1;
+  const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
+  const T iedx_q = g.iedx[qg];
+  const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  /* reciprocal spacings, used in place of quotients on power-of-two meshes */
+  const T rcdx_q = T(1.0) / cdx_q;
+  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+
+  /* x stage: x flux in y at corner row gf from u0 rows gf-2 .. gf+1 */
+  auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
+    const T v_cell = T(0.5) * (up(a_v, 1) + a_v);
+    return momentum_flux(edx_q, a_fy, up(a_fy, 1), w0, w1, w2, w3, iedy(gf),
+                         v_cell, cdy(gf - 1), cdy(gf), dt_h);
+  };
+  /* x stage: u1 of cell row i (ghost faces: the negated mirror) */
+  auto urow = [&](int i, T a_m1, T a_r, T uf_hi, T uf_lo, T& m1) {
+    m1 = advect_update<T, P2>(a_m1, dt_h, uf_hi, uf_lo, cdx_q, edy(i),
+                              rcdx_q * iedy(i));
+    const T rex = rho_edge_x<T, P2>(a_r, up(a_r, 1), cdx_q, cdx_qm, cdy(i), edx_q,
+                                    T(0.5) * iedx_q * rcdy(i));
+    T u1 = velocity(m1, rex);
+    return mirror_fix(xedge, u1, mf_lane, face_ghost, T(-1.0));
+  };
+  /* y stage: y flux in y at cell row i from v0 rows i-1 .. i+2 */
+  auto cellflux = [&](int i, T w0, T w1, T w2, T w3, T fy_lo, T fy_hi) {
+    return momentum_flux(cdx_q, fy_lo, fy_hi, w0, w1, w2, w3, iedy(i),
+                         T(0.5) * (w1 + w2), cdy(i - 1), cdy(i), dt_h);
+  };
+
+  /* prologue: u0 and v0 rows y0-2 .. y0+1 (ghost rows are in memory) */
+  T wu[4], wv[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    wu[s] = io.u0[(y0 - 2 + s) * sx + qf];
+    wv[s] = io.v0[(y0 - 2 + s) * nx + qc];
+  }
+  const T fy0 = io.F_y[y0 * nx + qc];
+  T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], wv[2], fy0);
+  const bool south = (y0 == kPad);
+  T vfy_prev = south ? T(0)
+                     : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
+                                io.F_y[(y0 - 1) * nx + qc], fy0);
+  T p_fx = io.F_x[(y0 - 1) * sx + qf], p_r = io.rho[(y0 - 1) * nx + qc];
+  /* u of cell row y0-1 for the y stage's first row: the x stage there
+   * (below the south wall it is row y0's, taken in the first step) */
+  T p_u = T(0);
+  if (!south) {
+    const T um3 = io.u0[(y0 - 3) * sx + qf];
+    const T uf_m = corner(y0 - 1, um3, wu[0], wu[1], wu[2], wv[1],
+                          io.F_y[(y0 - 1) * nx + qc]);
+    T m1;
+    p_u = urow(y0 - 1, io.mx1[(y0 - 1) * sx + qf], p_r, uf_prev, uf_m, m1);
+  }
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    wu[s] = wu[s + 1];
+    wv[s] = wv[s + 1];
+  }
+  T p_fy = fy0;
+  T a_u = io.u0[min(y0 + 2, ny - 1) * sx + qf], a_v = io.v0[(y0 + 2) * nx + qc];
+  T a_fy = io.F_y[(y0 + 1) * nx + qc];
+  T a_mx = io.mx1[y0 * sx + qf], a_my = io.my1[y0 * nx + qc];
+  T a_r = io.rho[y0 * nx + qc], a_fx = io.F_x[y0 * sx + qf];
+  for (int r = y0; r < gend; ++r) {
+    wu[3] = a_u;
+    wv[3] = a_v;
+    const T c_fy = a_fy, c_mx = a_mx, c_my = a_my, c_r = a_r, c_fx = a_fx;
+    {
+      const int rc = min(r + 1, ny - 1);
+      a_u = io.u0[min(r + 3, ny - 1) * sx + qf];
+      a_v = io.v0[min(r + 3, ny) * nx + qc];
+      a_fy = io.F_y[min(r + 2, ny) * nx + qc];
+      a_mx = io.mx1[rc * sx + qf];
+      a_my = io.my1[min(r + 1, ny) * nx + qc];
+      a_r = io.rho[rc * nx + qc];
+      a_fx = io.F_x[rc * sx + qf];
+    }
+    /* x stage: cell row r (past the north wall, row y1 is a ghost row:
+     * its u is row y1-1's) */
+    T u1 = p_u;
+    if (r < y1) {
+      const T uf = corner(r + 1, wu[0], wu[1], wu[2], wu[3], wv[2], c_fy);
+      T m1;
+      u1 = urow(r, c_mx, c_r, uf, uf_prev, m1);
+      const T up1 = dn(u1, 1);
+      T ufx = momentum_flux(edy(r), c_fx, dn(c_fx, 1), up(u1, 1), u1, up1,
+                            dn(u1, 2), iedx_q, T(0.5) * (u1 + up1), cdx_qm, cdx_q,
+                            dt_h);
+      ufx = mirror_fix(xedge, ufx, mc_lane, cell_ghost, T(1.0));
+      const T ufx_lo = up(ufx, 1);
+      if (own_x) {
+        io.mom_x[r * sx + q] = advect_update<T, P2>(m1, dt_h, ufx, ufx_lo, edx_q,
+                                                    cdy(r), iedx_q * rcdy(r));
+        mput_xface(edge, io.u, nx, ny, r, q, u1, T(-1.0));
+      }
+      uf_prev = uf;
+    }
+    if (south && r == y0) p_u = u1;
+    /* y stage: y-face row r */
+    const T vfy = (r >= ny - kPad)
+                      ? vfy_prev
+                      : cellflux(r, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
+    if (south && r == y0) vfy_prev = vfy;
+    const T m1y = advect_update<T, P2>(c_my, dt_h, vfy, vfy_prev, cdx_q, edy(r),
+                                       rcdx_q * iedy(r));
+    const T rey = rho_edge_y<T, P2>(c_r, p_r, cdx_q, cdy(r), cdy(r - 1), edy(r),
+                                    T(0.5) * rcdx_q * iedy(r));
+    T v1 = velocity(m1y, rey);
+    v1 = mirror_fix(xedge, v1, mc_lane, cell_ghost, T(1.0));
+    const T vm1 = up(v1, 1);
+    const T vfx = momentum_flux(cdy(r), c_fx, p_fx, up(v1, 2), vm1, v1, dn(v1, 1),
+                                iedx_q, T(0.5) * (u1 + p_u), cdx_qm, cdx_q, dt_h);
+    const T vfx_hi = dn(vfx, 1);
+    if (own_y) {
+      io.mom_y[r * nx + q] = advect_update<T, P2>(m1y, dt_h, vfx_hi, vfx, edx_q,
+                                                  cdy(r), iedx_q * rcdy(r));
+      mput_yface(edge, io.v, nx, ny, r, q, v1, T(-1.0));
+    }
+    vfy_prev = vfy;
+    p_fy = c_fy;
+    p_fx = c_fx;
+    p_u = u1;
+    p_r = c_r;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      wu[s] = wu[s + 1];
+      wv[s] = wv[s + 1];
+    }
+  }
+}
+
+template <typename T, bool P2>
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
+    MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
+  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves)
+    momxy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
+}
+
+template <typename T, bool P2>
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
+    MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
+  const T dt = T(dtp[0]), dt_h = T(dtp[1]);
+  const int lane = threadIdx.x % kWave;
+  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+       item += gridDim.x * kMWaves)
+    momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
+}
+
 template <typename K>
 int march_grid(K kernel, int& per_cu, int nitems) {
   static int cus = 0;

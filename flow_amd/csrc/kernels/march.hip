@@ -74,6 +74,29 @@ hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
   return hipGetLastError();
 }
 
+/* both momentum directions in one march: strips of 59 columns */
+constexpr int kMomXYW = kWave - 5;
+
+template <typename T, bool P2>
+hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
+                                int chunk_rows, hipStream_t s) {
+  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
+  const MarchList ml = make_march(g.nx, g.ny, kMomXYW, chunk_rows);
+  if (ml.nitems < 1) return hipErrorInvalidValue;
+  if (tt % 2) {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momxy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
+                       dtp, ml);
+  } else {
+    static int per_cu = 0;
+    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems);
+    hipLaunchKernelGGL((k_march_momxy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
+                       dtp, ml);
+  }
+  return hipGetLastError();
+}
+
 /* Dispatch on whether every mesh spacing is a power of two (quotients by
  * spacings become products with exact reciprocals). */
 template <typename T>
@@ -101,6 +124,13 @@ hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
               : launch_march_momentum_p<T, false>(g, io, tt, ydir, dtp, chunk_rows, s);
 }
 
+template <typename T>
+hipError_t launch_march_momxy(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
+                              int chunk_rows, int pow2, hipStream_t s) {
+  return pow2 ? launch_march_momxy_p<T, true>(g, io, tt, dtp, chunk_rows, s)
+              : launch_march_momxy_p<T, false>(g, io, tt, dtp, chunk_rows, s);
+}
+
 }  // namespace flowhip
 
 using namespace flowhip;
@@ -121,6 +151,15 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
   MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
   return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
                                          chunk_rows, m.pow2, s);
+}
+
+hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
+                             const double* F_x, const double* F_y, const double* rho,
+                             const double* mx1, const double* my1, double* mom_x,
+                             double* mom_y, double* u, double* v, const double* dtp,
+                             int chunk_rows, hipStream_t s) {
+  MMom2IO<double> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
+  return launch_march_momxy<double>(mgeo64(m), io, tt, dtp, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
@@ -174,6 +213,15 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
                                     int chunk_rows, hipStream_t s) {
   MMassIO<float> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
   return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, s);
+}
+
+hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,
+                                 const float* F_x, const float* F_y, const float* rho,
+                                 const float* mx1, const float* my1, float* mom_x,
+                                 float* mom_y, float* u, float* v, const double* dtp,
+                                 int chunk_rows, hipStream_t s) {
+  MMom2IO<float> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
+  return launch_march_momxy<float>(mgeo32(m), io, tt, dtp, chunk_rows, m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,

@@ -183,6 +183,14 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
                                double* my1, double* u0, double* v0, double* e1,
                                double* rho_old, FkReduce r, double max_dt,
                                int first_step, int chunk_rows, hipStream_t s);
+/* omp3/flow.c:557-775: both momentum directions in one row march (the y
+ * direction reads the u the x direction has just computed in the same
+ * wave); the result equals fk_march_momentum_x then _y. */
+hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
+                             const double* F_x, const double* F_y, const double* rho,
+                             const double* mx1, const double* my1, double* mom_x,
+                             double* mom_y, double* u, double* v, const double* dtp,
+                             int chunk_rows, hipStream_t s);
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
                                const double* v0, const double* F_x,
                                const double* F_y, const double* rho,
@@ -214,6 +222,11 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
                                     const float* e1, float* rho, float* e,
                                     float* F_x, float* F_y, FkReduce r,
                                     int chunk_rows, hipStream_t s);
+hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,
+                                 const float* F_x, const float* F_y, const float* rho,
+                                 const float* mx1, const float* my1, float* mom_x,
+                                 float* mom_y, float* u, float* v, const double* dtp,
+                                 int chunk_rows, hipStream_t s);
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
                                    const float* v0, const float* F_x,
                                    const float* F_y, const float* rho,
