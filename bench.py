@@ -199,7 +199,7 @@ def ensemble_leg(args):
         ens = flow_amd.Ensemble(sizes, dt=args.ens_dt, fp32=fp32)
         for k, sp in enumerate(specs):
             ens.set_state(k, *problems.ensemble_instance(*sp))
-        (rows, order), tuned_ms = ens.autotune(trial_steps=3)
+        (rows, order, fused), tuned_ms = ens.autotune(trial_steps=3)
         ens.set_graphs(True)
         dts, mass, energy, wall = ens.run(args.ens_steps)
         ok = bool(np.all(np.isfinite(dts)) and np.all(dts[:, :, 0] > 0))
@@ -210,7 +210,7 @@ def ensemble_leg(args):
             "cell_updates_per_s": cells * args.ens_steps / wall,
             "wall_s": wall, "ms_per_step": wall / args.ens_steps * 1e3,
             "autotuned": {"chunk_rows": rows, "work_order": order,
-                          "trial_ms_per_step": tuned_ms},
+                          "fused_momentum": fused, "trial_ms_per_step": tuned_ms},
             "finite": ok,
             "dt_spread": [float(dts[:, -1, 0].min()), float(dts[:, -1, 0].max())],
         }

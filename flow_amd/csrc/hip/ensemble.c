@@ -41,6 +41,7 @@ struct FlowHipEnsemble {
   int nwork;
   int chunk_rows, order;
   int pow2; /* every instance's spacings are powers of two */
+  int fused_momentum;
   FkEnsPartials part;
   int part_items;
   FkEnsHistory hist;
@@ -143,6 +144,7 @@ static FkEnsLaunch ens_launch(const FlowHipEnsemble* E) {
   L.nwork = E->nwork;
   L.chunk_rows = E->chunk_rows;
   L.pow2 = E->pow2;
+  L.fused_momentum = E->fused_momentum;
   L.part = E->part;
   L.hist = E->hist;
   return L;
@@ -162,8 +164,9 @@ FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
   E->ninst = ninst;
   E->fp32 = fp32 ? 1 : 0;
   E->esize = fp32 ? (int)sizeof(float) : (int)sizeof(double);
-  E->chunk_rows = 32;
+  E->chunk_rows = 48;
   E->order = ENS_ORDER_LARGEST;
+  E->fused_momentum = 1;
   E->gnx = (int*)malloc(sizeof(int) * (size_t)ninst);
   E->gny = (int*)malloc(sizeof(int) * (size_t)ninst);
   E->h_insts = (FkEnsInst*)calloc((size_t)ninst, sizeof(FkEnsInst));
@@ -370,19 +373,23 @@ int flowhip_ensemble_set_graphs(FlowHipEnsemble* E, int on) {
   return 0;
 }
 
-int flowhip_ensemble_set_schedule(FlowHipEnsemble* E, int chunk_rows, int order) {
+int flowhip_ensemble_set_schedule(FlowHipEnsemble* E, int chunk_rows, int order,
+                                  int fused_momentum) {
   if (!E || chunk_rows < 4 || chunk_rows > 120 || order < 0 || order > 2) return -1;
   HIPCHECK(hipStreamSynchronize(E->stream));
   E->chunk_rows = chunk_rows;
   E->order = order;
+  E->fused_momentum = fused_momentum ? 1 : 0;
   ens_build_work(E);
   return 0;
 }
 
-int flowhip_ensemble_get_schedule(FlowHipEnsemble* E, int* chunk_rows, int* order) {
+int flowhip_ensemble_get_schedule(FlowHipEnsemble* E, int* chunk_rows, int* order,
+                                  int* fused_momentum) {
   if (!E) return -1;
   if (chunk_rows) *chunk_rows = E->chunk_rows;
   if (order) *order = E->order;
+  if (fused_momentum) *fused_momentum = E->fused_momentum;
   return 0;
 }
 
@@ -481,9 +488,10 @@ int flowhip_ensemble_run(FlowHipEnsemble* E, int niters, int tt0, int pre_timest
   return niters;
 }
 
-/* Try every chunk height in {16, 32, 64} and work order (instance,
- * largest first, round robin) for trial_steps steps from the current
- * state; keep the fastest and put the state back. */
+/* Try every chunk height in {16, 32, 48, 64}, work order (instance,
+ * largest first, round robin) and momentum schedule (fused, x then y) for
+ * trial_steps steps from the current state; keep the fastest and put the
+ * state back. */
 int flowhip_ensemble_autotune(FlowHipEnsemble* E, int trial_steps, int tt0,
                               double* best_ms) {
   if (!E || trial_steps < 1) return -1;
@@ -493,17 +501,19 @@ int flowhip_ensemble_autotune(FlowHipEnsemble* E, int trial_steps, int tt0,
   HIPCHECK(hipMemcpy(E->snap, E->pool, E->pool_bytes, hipMemcpyDeviceToDevice));
   HIPCHECK(hipMemcpy((char*)E->snap + E->pool_bytes, E->d_dtp,
                      sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToDevice));
-  const int rows[3] = {16, 32, 64};
+  const int rows[4] = {16, 32, 48, 64};
   double best = 1e300;
-  int brow = E->chunk_rows, bord = E->order;
+  int brow = E->chunk_rows, bord = E->order, bfm = E->fused_momentum;
   hipEvent_t a, b;
   HIPCHECK(hipEventCreate(&a));
   HIPCHECK(hipEventCreate(&b));
   ens_reserve_history(E, trial_steps + 1);
-  for (int r = 0; r < 3; ++r)
-    for (int o = 0; o < 3; ++o) {
+  for (int r = 0; r < 4; ++r)
+    for (int o = 0; o < 3; ++o)
+    for (int fm = 0; fm < 2; ++fm) {
       E->chunk_rows = rows[r];
       E->order = o;
+      E->fused_momentum = fm;
       ens_build_work(E);
       HIPCHECK(fk_ens_set_counter(E->hist.counter, 0, E->stream));
       /* one untimed step, then the timed ones */
@@ -519,6 +529,7 @@ int flowhip_ensemble_autotune(FlowHipEnsemble* E, int trial_steps, int tt0,
         best = ms;
         brow = rows[r];
         bord = o;
+        bfm = fm;
       }
       HIPCHECK(hipMemcpyAsync(E->pool, E->snap, E->pool_bytes, hipMemcpyDeviceToDevice,
                               E->stream));
@@ -531,6 +542,7 @@ int flowhip_ensemble_autotune(FlowHipEnsemble* E, int trial_steps, int tt0,
   HIPCHECK(hipEventDestroy(b));
   E->chunk_rows = brow;
   E->order = bord;
+  E->fused_momentum = bfm;
   ens_build_work(E);
   if (best_ms) *best_ms = best / trial_steps;
   return 0;

@@ -23,7 +23,9 @@
 namespace flowhip {
 
 /* Strip width of every ensemble kernel: the narrowest any march kernel
- * allows (the Lagrangian kernel's), so one work list serves all stages. */
+ * allows (the Lagrangian kernel's), so one work list serves all stages.
+ * Momentum runs as one fused march or as the x then y marches, a schedule
+ * choice of the host (and its autotuner). */
 constexpr int kEnsW = kWave - 6;
 
 template <typename T>
@@ -161,6 +163,27 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
   }
 }
 
+/* Both momentum directions in one march (momxy_*_item). */
+template <typename T, bool P2>
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
+    const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    int H, int tt) {
+  const int lane = threadIdx.x % kWave;
+  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+    const int2 wi = work[w];
+    const FkEnsInst& in = insts[wi.x];
+    const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
+    const MMom2IO<T> io{arr<T>(in, kUFx), arr<T>(in, kUFy), arr<T>(in, kFx),
+                        arr<T>(in, kFy),  arr<T>(in, kRho), arr<T>(in, kQxx),
+                        arr<T>(in, kQyy), arr<T>(in, kMomX), arr<T>(in, kMomY),
+                        arr<T>(in, kU),   arr<T>(in, kV)};
+    if (tt % 2)
+      momxy_xy_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane);
+    else
+      momxy_yx_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane);
+  }
+}
+
 /* CFL candidates of one item's cells (the pre-loop set_timestep). */
 template <typename T>
 __global__ __launch_bounds__(kMNT) void k_ens_timestep(
@@ -286,11 +309,18 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
                      dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, L.hist);
-  const int gm = march_grid(k_ens_momentum<T, P2>, pc_mom, nw);
-  hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
-                     L.chunk_rows, tt, 0);
-  hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts, work, nw,
-                     L.chunk_rows, tt, 1);
+  if (L.fused_momentum) {
+    static int pc_momxy = 0;
+    hipLaunchKernelGGL((k_ens_momentum_xy<T, P2>),
+                       dim3(march_grid(k_ens_momentum_xy<T, P2>, pc_momxy, nw)),
+                       dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt);
+  } else {
+    const int gm = march_grid(k_ens_momentum<T, P2>, pc_mom, nw);
+    hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts,
+                       work, nw, L.chunk_rows, tt, 0);
+    hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts,
+                       work, nw, L.chunk_rows, tt, 1);
+  }
   hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(1), 0, s, L.hist.counter);
   return hipGetLastError();
 }

@@ -89,8 +89,8 @@ def lib():
         "flowhip_ensemble_get_array": (i, [vp, i, i, vp]),
         "flowhip_ensemble_set_dt": (i, [vp, i, d, d]),
         "flowhip_ensemble_set_graphs": (i, [vp, i]),
-        "flowhip_ensemble_set_schedule": (i, [vp, i, i]),
-        "flowhip_ensemble_get_schedule": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "flowhip_ensemble_set_schedule": (i, [vp, i, i, i]),
+        "flowhip_ensemble_get_schedule": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
         "flowhip_ensemble_autotune": (i, [vp, i, i, _dp]),
         "flowhip_ensemble_run": (i, [vp, i, i, i, vp, vp, vp, _dp]),
         "allocate_data": (sz, [ctypes.POINTER(vp), sz]),
@@ -386,14 +386,16 @@ class Ensemble:
     def set_graphs(self, on):
         lib().flowhip_ensemble_set_graphs(self.ptr, 1 if on else 0)
 
-    def set_schedule(self, chunk_rows, order):
-        if lib().flowhip_ensemble_set_schedule(self.ptr, chunk_rows, order) != 0:
+    def set_schedule(self, chunk_rows, order, fused_momentum=True):
+        if lib().flowhip_ensemble_set_schedule(self.ptr, chunk_rows, order,
+                                               1 if fused_momentum else 0) != 0:
             raise ValueError("bad schedule")
 
     def schedule(self):
-        a, b = ctypes.c_int(), ctypes.c_int()
-        lib().flowhip_ensemble_get_schedule(self.ptr, ctypes.byref(a), ctypes.byref(b))
-        return a.value, b.value
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().flowhip_ensemble_get_schedule(self.ptr, ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(c))
+        return a.value, b.value, bool(c.value)
 
     def autotune(self, trial_steps=3, tt0=0):
         ms = ctypes.c_double()

@@ -111,12 +111,15 @@ int flowhip_ensemble_get_array(FlowHipEnsemble* ens, int inst, int which,
                                double* host);
 int flowhip_ensemble_set_dt(FlowHipEnsemble* ens, int inst, double dt, double dt_h);
 int flowhip_ensemble_set_graphs(FlowHipEnsemble* ens, int on);
-/* Chunk height (4..120) and work order (0: by instance, 1: largest
- * instance first, 2: one chunk row of every instance in turn). */
-int flowhip_ensemble_set_schedule(FlowHipEnsemble* ens, int chunk_rows, int order);
-int flowhip_ensemble_get_schedule(FlowHipEnsemble* ens, int* chunk_rows, int* order);
-/* Times trial_steps steps of each chunk height and work order from the
- * current state, keeps the fastest, restores the state. */
+/* Chunk height (4..120), work order (0: by instance, 1: largest instance
+ * first, 2: one chunk row of every instance in turn) and momentum schedule
+ * (1: both directions in one march, 0: x then y). */
+int flowhip_ensemble_set_schedule(FlowHipEnsemble* ens, int chunk_rows, int order,
+                                  int fused_momentum);
+int flowhip_ensemble_get_schedule(FlowHipEnsemble* ens, int* chunk_rows, int* order,
+                                  int* fused_momentum);
+/* Times trial_steps steps of each chunk height, work order and momentum
+ * schedule from the current state, keeps the fastest, restores the state. */
 int flowhip_ensemble_autotune(FlowHipEnsemble* ens, int trial_steps, int tt0,
                               double* best_ms_per_step);
 /* niters steps of every instance; dt_pairs [inst][niters+1][2], mass and

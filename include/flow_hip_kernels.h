@@ -271,6 +271,7 @@ typedef struct FkEnsLaunch {
   const int* work;         /* device, nwork (instance, item) pairs */
   int ninst, nwork, chunk_rows;
   int pow2;                /* every instance's spacings are powers of two */
+  int fused_momentum;      /* both momentum directions in one march */
   FkEnsPartials part;
   FkEnsHistory hist;
 } FkEnsLaunch;

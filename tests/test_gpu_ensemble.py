@@ -75,7 +75,8 @@ def test_ensemble_instances_equal_single_runs_fp32():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("schedule", [(16, 0), (64, 2), (120, 1)])
+@pytest.mark.parametrize("schedule", [(16, 0, True), (64, 2, False), (120, 1, True),
+                                      (48, 1, False)])
 def test_ensemble_schedule_does_not_change_results(schedule):
     print(f"case ensemble schedule={schedule}", flush=True)
     ref = _ensemble()
