@@ -60,6 +60,19 @@ struct MarchList {
   int nstrips, nchunks, nitems;
 };
 
+/* A wave's first work item. march_first_uniform makes it wave-uniform:
+ * the item, its bounds and the row counter then live in scalar registers,
+ * row tests become scalar branches and per-row geometry is read with a
+ * scalar lane index. That pays in the Lagrangian and mass/energy marches;
+ * in the momentum marches the extra scalar registers it costs made them
+ * slower (measured on MI355X), so those keep the per-lane index. */
+__device__ __forceinline__ int march_first() {
+  return blockIdx.x * kMWaves + threadIdx.x / kWave;
+}
+__device__ __forceinline__ int march_first_uniform() {
+  return __builtin_amdgcn_readfirstlane(march_first());
+}
+
 inline MarchList make_march(int nx, int ny, int W, int H) {
   MarchList m;
   m.W = W;
@@ -72,14 +85,15 @@ inline MarchList make_march(int nx, int ny, int W, int H) {
 
 /* Whole-wave shifts by one lane as DPP moves (wave_shr:1 / wave_shl:1):
  * VALU operations on registers, with none of the LDS round trip of a
- * ds_bpermute. Lane 0 of a shift up and lane 63 of a shift down keep their
- * own value; no kernel reads those lanes' results. */
+ * ds_bpermute. Lane 0 of a shift up and lane 63 of a shift down read zero
+ * (bound_ctrl), so a move needs no copy of its destination's old value; no
+ * kernel reads those lanes' results. */
 constexpr int kDppWaveShl1 = 0x130; /* lane i <- lane i+1 */
 constexpr int kDppWaveShr1 = 0x138; /* lane i <- lane i-1 */
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_move(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
@@ -332,7 +346,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first_uniform(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
@@ -468,7 +482,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first_uniform(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
@@ -616,7 +630,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     momx_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
@@ -712,7 +726,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     momx_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
@@ -856,7 +870,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     momy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
@@ -961,7 +975,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     momy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
@@ -1214,7 +1228,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   constexpr bool kSq = std::is_same<T, float>::value;
   double local_min = kSq ? max_dt * max_dt : max_dt;
   int nan_count = 0;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first_uniform(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
     lagrangian_item<T, P2>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
@@ -1597,7 +1611,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves)
     momxy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
 }
@@ -1607,7 +1621,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = blockIdx.x * kMWaves + threadIdx.x / kWave; item < ml.nitems;
+  for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves)
     momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
 }
