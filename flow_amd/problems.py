@@ -69,6 +69,13 @@ def square_blast(n):
                         (1.0, 100.0 / (GAMMA - 1.0), 0.45, 0.45, 0.1, 0.1)])
 
 
+def blast_4096(n):
+    """The 4096^2 blast workload: rho=1, e=0.25 background; a centre box of
+    side 0.1 at e=250 (409^2 cells at n=4096 by the left-edge rule)."""
+    return boxes(n, n, [(1.0, 0.25, 0.0, 0.0, 1.0, 1.0),
+                        (1.0, 250.0, 0.45, 0.45, 0.1, 0.1)])
+
+
 def flow_params_problem(n):
     """The problem of the reference's flow.params (flow.params:1-2)."""
     return boxes(n, n, [(0.125, 2.0, 0.0, 0.0, 1.0, 1.0),

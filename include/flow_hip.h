@@ -136,6 +136,24 @@ int flowhip_copy(double* dev, double* host, size_t len, int dir);
 /* Name of the GPU in use. */
 int flowhip_device_name(char* name, int name_len);
 
+/* Bit-exact state files (format in flow_amd/csrc/hip/state.c): the carried
+ * fields rho, e, u, v, mom_x and mom_y (device arrays), the timestep pair,
+ * the steps completed and the simulated time. The reference has no
+ * counterpart; its driver keeps no state between runs. A file with step 0
+ * is an initial state; a later step is a restart point that continues the
+ * run bit for bit (mesh->dt and dt_h are restored on load). Both return 0,
+ * or -1 after printing what failed (an unreadable file, another mesh size
+ * or domain). */
+int flowhip_state_save(const char* path, const Mesh* mesh, int step, double sim_time,
+                       const double* rho, const double* e, const double* u,
+                       const double* v, const double* mom_x, const double* mom_y);
+int flowhip_state_load(const char* path, Mesh* mesh, int* step, double* sim_time,
+                       double* rho, double* e, double* u, double* v, double* mom_x,
+                       double* mom_y);
+/* Header fields of a state file (any output may be NULL). */
+int flowhip_state_info(const char* path, int* global_nx, int* global_ny, int* step,
+                       double* sim_time);
+
 /* Mesh objects for callers that cannot place a Mesh on their stack: reads
  * no files; width/height/max_dt/sim_end are the arch.params values and
  * dt_param the .params dt (main.c:20-31). */

@@ -1,0 +1,69 @@
+"""The named workloads at their own sizes, on the default (march) schedule.
+
+- Sod 512^2, 100 steps: bitwise equal to the oracle over the carried state,
+  the dt sequence and the totals.
+- The 4096^2 blast: the first 20 steps bitwise equal to the oracle (what the
+  oracle finishes in seconds); the full 500 steps on the GPU checked through
+  properties that hold at any size: interior mass conserved to round-off
+  (reflective walls pass no flux), every field and dt finite and positive
+  where it must be, and two runs bitwise identical.
+"""
+import numpy as np
+import pytest
+
+import flow_amd
+from flow_amd import problems
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+
+def _bitwise(got, ref, sum_rtol=1e-13):
+    """Fields and dt bit for bit; the totals to sum_rtol. The oracle's totals
+    are plain FP64 sums in OpenMP reduction order, the kernel set's are
+    compensated: over 16.8M cells (4096^2) the plain sum's own rounding
+    reaches 1e-13 of the total."""
+    L = got["L"]
+    for k, (rel, ndiff) in harness.compare_state(got["arrays"], ref["arrays"], L).items():
+        print(f"  {k:6s} rel_linf={rel:.3e} bitwise_diffs={ndiff}", flush=True)
+        assert ndiff == 0, (k, rel, ndiff)
+    assert got["steps"] == ref["steps"]
+    np.testing.assert_array_equal(got["dt"], ref["dt"])
+    np.testing.assert_allclose(got["mass"], ref["mass"], rtol=sum_rtol)
+    np.testing.assert_allclose(got["energy"], ref["energy"], rtol=sum_rtol)
+
+
+def test_sod_512_100_steps_bitwise():
+    print("case sod_x n=512 steps=100 march", flush=True)
+    ref = harness.oracle_run(problems.sod_x, 512, 100, dt=1e-4)
+    got = harness.gpu_run(problems.sod_x, 512, 100, flow_amd.MARCH, dt=1e-4)
+    _bitwise(got, ref)
+
+
+def test_blast_4096_first_20_steps_bitwise():
+    print("case blast_4096 n=4096 steps=20 march", flush=True)
+    ref = harness.oracle_run(problems.blast_4096, 4096, 20, dt=1e-4)
+    got = harness.gpu_run(problems.blast_4096, 4096, 20, flow_amd.MARCH, dt=1e-4)
+    _bitwise(got, ref, sum_rtol=1e-12)
+
+
+def test_blast_4096_500_steps_invariants():
+    print("case blast_4096 n=4096 steps=500 march (twice)", flush=True)
+    a = harness.gpu_run(problems.blast_4096, 4096, 500, flow_amd.MARCH, dt=1e-4)
+    assert a["steps"] == 500
+    dt = np.asarray(a["dt"])
+    assert np.all(np.isfinite(dt)) and np.all(dt > 0)
+    mass = np.asarray(a["mass"])
+    drift = np.max(np.abs(mass - mass[0])) / mass[0]
+    print(f"  mass drift over 500 steps {drift:.3e}", flush=True)
+    assert drift <= 1e-12
+    L = a["L"]
+    v = harness.views(a["arrays"], L)
+    reg = harness.state_regions(L)
+    for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
+        assert np.all(np.isfinite(v[k][reg[k]])), k
+    assert np.all(v["rho"][2:L - 2, 2:L - 2] > 0)
+    b = harness.gpu_run(problems.blast_4096, 4096, 500, flow_amd.MARCH, dt=1e-4)
+    for k, (rel, ndiff) in harness.compare_state(a["arrays"], b["arrays"], L).items():
+        assert ndiff == 0, (k, ndiff)
+    np.testing.assert_array_equal(a["dt"], b["dt"])

@@ -1,0 +1,122 @@
+"""flow.hip, the kernel set's own driver (flow's main.c counterpart), on the
+GPU: flow.params' known answer through the same files and printed lines as
+main.c, a non-zero exit when validation fails, --dump equal to the oracle's
+state bit for bit, and --load continuing a run bit for bit."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from flow_amd import problems
+from flow_amd import statefile as sf
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(os.path.dirname(HERE), "flow_amd", "lib", "flow.hip")
+KAT = json.load(open(os.path.join(HERE, "golden", "flow_tests_kat.json")))
+
+
+def _case(tmp_path, n, iterations, dt, boxes, tests=None):
+    """A run directory with flow.params (and flow.tests) below arch.params."""
+    (tmp_path / "arch.params").write_text(
+        "width 1.0\nheight 1.0\nmax_dt 1.0e30\nsim_end 1.0e30\n")
+    run = tmp_path / "flow"
+    run.mkdir(exist_ok=True)
+    lines = [f"problem_{k} density={b[0]!r} energy={b[1]!r} xpos={b[2]!r} "
+             f"ypos={b[3]!r} width={b[4]!r} height={b[5]!r}" for k, b in enumerate(boxes)]
+    lines += [f"dt {dt!r}", f"nx {n}", f"ny {n}", f"iterations {iterations}",
+              "visit_dump 0"]
+    (run / "flow.params").write_text("\n".join(lines) + "\n")
+    if tests:
+        (run / "flow.tests").write_text(tests)
+    return run
+
+
+def _drive(run, *args, timeout=110):
+    r = subprocess.run([DRIVER, "flow.params", *args], cwd=run, capture_output=True,
+                       text=True, timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
+
+
+def test_driver_known_answer_passes(tmp_path):
+    print(f"case flow.hip known answer n={KAT['nx']}", flush=True)
+    tests = (f"flow.params density={KAT['density']:.12e} "
+             f"energy={KAT['energy']:.12e}\n")
+    run = _case(tmp_path, KAT["nx"], KAT["iterations"], KAT["dt"], KAT["problems"], tests)
+    rc, out, err = _drive(run)
+    assert rc == 0, out[-2000:] + err[-2000:]
+    assert "PASSED validation." in out
+    assert out.count("\nIteration ") == KAT["iterations"]
+    assert out.count("Total mass:") == KAT["iterations"]
+
+
+def test_driver_failed_validation_exits_nonzero(tmp_path):
+    print("case flow.hip failed validation n=64", flush=True)
+    tests = "flow.params density=1.0e+00 energy=1.0e+00\n"
+    run = _case(tmp_path, 64, 3, 1e-4, KAT["problems"], tests)
+    rc, out, err = _drive(run)
+    assert rc == 1, out[-2000:] + err[-2000:]
+    assert "FAILED validation." in out
+
+
+def test_driver_dump_equals_oracle(tmp_path):
+    n, steps, dt = 64, 6, 1e-4
+    print(f"case flow.hip --dump vs oracle n={n} steps={steps}", flush=True)
+    run = _case(tmp_path, n, steps, dt, KAT["problems"])
+    rc, out, err = _drive(run, "--dump", "final.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    assert "could NOT validate" in out
+    got = sf.read(run / "final.state")
+    assert got["step"] == steps
+    ref = harness.oracle_run(problems.flow_params_problem, n, steps, dt=dt)
+    L = n + 4
+    v = harness.views(ref["arrays"], L)
+    reg = harness.state_regions(L)
+    for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
+        a, b = got["arrays"][k][reg[k]], v[k][reg[k]]
+        assert np.count_nonzero(a != b) == 0, k
+    assert got["dt"] == ref["dt"][-1][0] and got["dt_h"] == ref["dt"][-1][1]
+
+
+def test_driver_restart_continues_bit_for_bit(tmp_path):
+    n, dt = 48, 1e-4
+    print(f"case flow.hip --load restart n={n} 3+3 steps vs 6", flush=True)
+    (tmp_path / "whole").mkdir()
+    whole = _case(tmp_path / "whole", n, 6, dt, KAT["problems"])
+    rc, out, err = _drive(whole, "--dump", "six.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    (tmp_path / "first").mkdir()
+    first = _case(tmp_path / "first", n, 3, dt, KAT["problems"])
+    rc, out, err = _drive(first, "--dump", "three.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    rc, out, err = _drive(first, "--load", "three.state", "--dump", "resumed.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    assert "\nIteration 4\n" in out and "\nIteration 6\n" in out
+    a, b = sf.read(first / "resumed.state"), sf.read(whole / "six.state")
+    assert a["step"] == b["step"] == 6
+    assert (a["dt"], a["dt_h"], a["sim_time"]) == (b["dt"], b["dt_h"], b["sim_time"])
+    L = n + 4
+    reg = harness.state_regions(L)
+    for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
+        assert a["arrays"][k][reg[k]].tobytes() == b["arrays"][k][reg[k]].tobytes(), k
+
+
+def test_driver_runs_a_generated_initial_state(tmp_path):
+    n, steps, dt = 40, 4, 1e-5
+    print(f"case flow.hip --load multi_blast initial state n={n}", flush=True)
+    run = _case(tmp_path, n, steps, dt, KAT["problems"])
+    rho, e = problems.multi_blast(n)
+    sf.write(run / "init.state", sf.initial_state(n, rho, e), n)
+    rc, out, err = _drive(run, "--load", "init.state", "--dump", "final.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    got = sf.read(run / "final.state")
+    ref = harness.oracle_run(None, n, steps, dt=dt, init=(rho, e))
+    L = n + 4
+    v = harness.views(ref["arrays"], L)
+    reg = harness.state_regions(L)
+    for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
+        assert np.count_nonzero(got["arrays"][k][reg[k]] != v[k][reg[k]]) == 0, k
