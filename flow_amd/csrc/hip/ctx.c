@@ -58,7 +58,7 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   memset(h->h_stage, 0, sizeof(double) * 8);
   memset(h->h_status, 0, sizeof(int) * 4);
   h->schedule = FLOWHIP_MARCH;
-  h->march_rows = 48;
+  h->march_rows = 0; /* each march kernel's own chunk height */
   schedule_from_env(h);
   /* FLOWHIP_POW2=0 keeps IEEE quotients by spacings even on power-of-two
    * meshes (the results are the same bits either way) */

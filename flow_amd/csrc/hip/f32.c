@@ -91,17 +91,18 @@ static void step_f32(FlowHipCtx* h, Mesh* mesh, FkMesh32 m, int tt, float** a) {
         *v = a[5], *mom_x = a[6], *mom_y = a[7], *Qxx = a[8], *Qyy = a[9],
         *F_x = a[10], *F_y = a[11], *uF_x = a[12], *uF_y = a[13];
   hipStream_t s = h->stream;
-  const int rows = h->march_rows;
   FK_LAUNCH(h, "lagrangian_f32",
             fk_march_lagrangian_f32(m, rho, e, mom_x, mom_y, Qxx, Qyy, uF_x,
                                     uF_y, P, rho_old, h->red, mesh->max_dt,
-                                    tt == 0, rows, s));
+                                    tt == 0, flowhip_rows(h, FLOWHIP_K_LAGRANGIAN, 1), s));
   FK_LAUNCH(h, "advect_mass_energy_f32",
             fk_march_mass_energy_f32(m, tt, uF_x, uF_y, rho_old, P, rho, e,
-                                     F_x, F_y, h->red, rows, s));
+                                     F_x, F_y, h->red, flowhip_rows(h, FLOWHIP_K_MASS, 1),
+                                     s));
   FK_LAUNCH(h, "advect_momentum_f32",
             fk_march_momentum_f32(m, tt, uF_x, uF_y, F_x, F_y, rho, Qxx, Qyy,
-                                  mom_x, mom_y, u, v, h->red.dtp, rows, s));
+                                  mom_x, mom_y, u, v, h->red.dtp,
+                                  flowhip_rows(h, FLOWHIP_K_MOMENTUM, 1), s));
 }
 
 static double now_seconds(void) {

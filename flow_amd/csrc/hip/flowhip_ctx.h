@@ -57,7 +57,7 @@ typedef struct FlowHipCtx {
   double* h_stage;      /* pinned: [0..1] dt pair, [2..5] sums */
   int* h_status;        /* pinned */
   int schedule;
-  int march_rows;
+  int march_rows;       /* chunk height of every march kernel; 0: each its own */
   int use_graphs;
   /* captured steps: [0] even tt, [1] odd tt, [2] tt == 0; each valid for
    * the launch arguments, schedule and chunk height it was captured with */
@@ -102,5 +102,16 @@ void flowhip_timer_collect(FlowHipCtx* h);
 
 /* Invalidate cached conservation sums (any write to device state). */
 void flowhip_invalidate_sums(Mesh* mesh);
+
+/* Chunk height of one march kernel. An explicit height (flowhip_set_march_rows,
+ * FLOWHIP_MARCH_ROWS) applies to all of them; by default each kernel uses
+ * the height measured fastest for it on MI355X at 8192^2 (it sets where
+ * chunk prologue rows and the last round of items over the resident waves
+ * balance; every height gives bitwise the same result). */
+enum { FLOWHIP_K_LAGRANGIAN = 0, FLOWHIP_K_MASS = 1, FLOWHIP_K_MOMENTUM = 2 };
+static inline int flowhip_rows(const FlowHipCtx* h, int kernel, int fp32) {
+  static const int kRows[2][3] = {{44, 48, 56}, {60, 48, 64}};
+  return h->march_rows > 0 ? h->march_rows : kRows[fp32 ? 1 : 0][kernel];
+}
 
 #endif

@@ -25,7 +25,8 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
     FK_LAUNCH(h, "lagrangian",
               fk_march_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1,
                                   my1, u0, v0, e1, rho0, h->red, st->max_dt,
-                                  st->tt == 0, h->march_rows, s));
+                                  st->tt == 0, flowhip_rows(h, FLOWHIP_K_LAGRANGIAN, 0),
+                                  s));
   else
     FK_LAUNCH(h, "lagrangian",
               fk_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1, my1,
@@ -33,7 +34,8 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
   if (h->schedule == FLOWHIP_MARCH)
     FK_LAUNCH(h, "advect_mass_energy",
               fk_march_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho, st->e,
-                                   st->F_x, st->F_y, h->red, h->march_rows, s));
+                                   st->F_x, st->F_y, h->red,
+                                   flowhip_rows(h, FLOWHIP_K_MASS, 0), s));
   else
     FK_LAUNCH(h, "advect_mass_energy",
               fk_advect_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho,
@@ -42,7 +44,7 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
     FK_LAUNCH(h, "advect_momentum",
               fk_march_momentum(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho, mx1,
                                 my1, st->mom_x, st->mom_y, st->u, st->v,
-                                h->red.dtp, h->march_rows, s));
+                                h->red.dtp, flowhip_rows(h, FLOWHIP_K_MOMENTUM, 0), s));
   } else {
     FK_LAUNCH(h, "advect_momentum_x",
               fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,

@@ -21,7 +21,7 @@ int flowhip_set_schedule(Mesh* mesh, int schedule) {
 int flowhip_get_schedule(Mesh* mesh) { return flowhip_ctx(mesh)->schedule; }
 
 int flowhip_set_march_rows(Mesh* mesh, int rows) {
-  if (rows < 4 || rows > 120) return -1;
+  if (rows != 0 && (rows < 4 || rows > 120)) return -1;
   flowhip_ctx(mesh)->march_rows = rows;
   return 0;
 }
