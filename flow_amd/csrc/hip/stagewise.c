@@ -183,4 +183,11 @@ void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {
     h->graph_rows[slot] = h->march_rows;
   }
   HIPCHECK(hipGraphLaunch(h->graph[slot], h->stream));
+  /* host-side state of a direct launch: the graph's advection kernel left
+   * the totals of the new rho and e on the device for print_conservation */
+  if (h->schedule == FLOWHIP_FUSED || h->schedule == FLOWHIP_MARCH) {
+    h->sums_valid = 1;
+    h->sums_rho = st->rho;
+    h->sums_e = st->e;
+  }
 }
