@@ -1626,8 +1626,11 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
 }
 
+/* Persistent grid of a march kernel: as many blocks as stay resident on the
+ * device (per_cu per CU, from the occupancy query on first use, at most
+ * max_per_cu when that is positive), and no more than the items need. */
 template <typename K>
-int march_grid(K kernel, int& per_cu, int nitems) {
+int march_grid(K kernel, int& per_cu, int nitems, int max_per_cu = 0) {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -1643,6 +1646,7 @@ int march_grid(K kernel, int& per_cu, int nitems) {
            hipSuccess ||
        per_cu < 1))
     per_cu = 1;
+  if (max_per_cu > 0 && per_cu > max_per_cu) per_cu = max_per_cu;
   long gsz = (long)cus * per_cu;
   const long need = (nitems + kMWaves - 1) / kMWaves;
   if (gsz > need) gsz = need;
