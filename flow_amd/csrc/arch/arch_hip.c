@@ -88,6 +88,11 @@ void handle_boundary_2d(const int nx, const int ny, Mesh* mesh, double* arr,
  * positions are only used for problem set-up and stay on the host; the
  * spacings are device arrays read by the kernels. */
 void initialise_mesh_2d(Mesh* mesh) {
+  if (!flowhip_mesh_size_ok(mesh->global_nx, mesh->global_ny))
+    TERMINATE("initialise_mesh_2d: a %dx%d mesh is outside what the hip kernel set "
+              "indexes (2..%d cells a side: every field's (n+5)^2 elements must "
+              "have 32-bit offsets)",
+              mesh->global_nx, mesh->global_ny, FLOWHIP_MAX_CELLS_SIDE);
   const int lx = mesh->local_nx + 1, ly = mesh->local_ny + 1;
   double* hx = (double*)calloc((size_t)lx, sizeof(double));
   double* hy = (double*)calloc((size_t)ly, sizeof(double));

@@ -158,7 +158,7 @@ FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
       !dt_param)
     return NULL;
   for (int i = 0; i < ninst; ++i)
-    if (global_nx[i] < 2 || global_ny[i] < 2) return NULL;
+    if (!flowhip_mesh_size_ok(global_nx[i], global_ny[i])) return NULL;
   FlowHipEnsemble* E = (FlowHipEnsemble*)calloc(1, sizeof(FlowHipEnsemble));
   if (!E) return NULL;
   E->ninst = ninst;

@@ -112,10 +112,15 @@ int flowhip_device_name(char* name, int name_len) {
   return 0;
 }
 
+int flowhip_mesh_size_ok(int global_nx, int global_ny) {
+  return global_nx >= 2 && global_ny >= 2 && global_nx <= FLOWHIP_MAX_CELLS_SIDE &&
+         global_ny <= FLOWHIP_MAX_CELLS_SIDE;
+}
+
 Mesh* flowhip_mesh_new(int global_nx, int global_ny, double width,
                        double height, double max_dt, double sim_end,
                        double dt_param) {
-  if (global_nx < 2 || global_ny < 2) return NULL;
+  if (!flowhip_mesh_size_ok(global_nx, global_ny)) return NULL;
   Mesh* m = (Mesh*)calloc(1, sizeof(Mesh));
   if (!m) return NULL;
   m->global_nx = global_nx;

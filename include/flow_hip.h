@@ -158,6 +158,15 @@ int flowhip_state_load(const char* path, Mesh* mesh, int* step, double* sim_time
 int flowhip_state_info(const char* path, int* global_nx, int* global_ny, int* step,
                        double* sim_time);
 
+/* Mesh sizes the kernel set can index: 2 to FLOWHIP_MAX_CELLS_SIDE interior
+ * cells a side. Kernels address a field's (n+5)^2 elements with 32-bit
+ * offsets; at n = 46335 one FP64 field is 17.2 GB and the 16 arrays of a
+ * step 275 GB, within the 288 GB of one MI355X. Larger meshes are refused
+ * (initialise_mesh_2d exits with a message, flowhip_mesh_new returns NULL)
+ * instead of overflowing an index. */
+#define FLOWHIP_MAX_CELLS_SIDE 46335
+int flowhip_mesh_size_ok(int global_nx, int global_ny);
+
 /* Mesh objects for callers that cannot place a Mesh on their stack: reads
  * no files; width/height/max_dt/sim_end are the arch.params values and
  * dt_param the .params dt (main.c:20-31). */

@@ -85,3 +85,23 @@ def test_library_refuses_to_run_without_build(tmp_path, monkeypatch):
     monkeypatch.setattr(native, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         native.lib()
+
+
+def test_mesh_sizes_outside_32_bit_indexing_are_refused():
+    """Fields are indexed with 32-bit element offsets: (n+5)^2 <= 2^31-1 up to
+    n = 46335. Larger meshes (and interiors below 2x2) are refused before any
+    device call, not left to overflow an index."""
+    import ctypes
+    from flow_amd import native
+    L = native.lib()
+    L.flowhip_mesh_size_ok.restype = ctypes.c_int
+    L.flowhip_mesh_size_ok.argtypes = [ctypes.c_int, ctypes.c_int]
+    assert (46335 + 5) ** 2 <= 2 ** 31 - 1 < (46336 + 5) ** 2
+    assert L.flowhip_mesh_size_ok(46335, 46335) == 1
+    assert L.flowhip_mesh_size_ok(2, 2) == 1
+    for nx, ny in ((46336, 46336), (46336, 8), (8, 46336), (1, 8), (8, 1), (0, 0)):
+        assert L.flowhip_mesh_size_ok(nx, ny) == 0, (nx, ny)
+    import pytest
+    import flow_amd
+    with pytest.raises(RuntimeError):
+        flow_amd.Mesh(46336, 46336)
