@@ -120,6 +120,13 @@ __device__ __forceinline__ T dn(T v, int d) {
 }
 
 
+/* Whether a work item's strip lies within three columns of a wall, where
+ * some of its lanes are ghost columns. */
+__device__ __forceinline__ bool item_xedge(const MarchList& ml, int nx, int item) {
+  const int x0 = kPad + (item % ml.nstrips) * ml.W, x1 = min(x0 + ml.W, nx - kPad);
+  return (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+}
+
 /* ---- per-row geometry of a chunk held across the lanes ----
  * Lane k holds rows base+k and base+64+k; row(k) broadcasts one row's value
  * with v_readlane (k is uniform), so the row loop issues no memory load for
@@ -1012,7 +1019,7 @@ struct MLagIO {
  * i+3 computed this step), Qyy on cell rows i and i+1 and v after the
  * viscous acceleration on faces i and i+1. Ghost faces hold the negated
  * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
-template <typename T, bool P2>
+template <typename T, bool P2, int XE = -1>
 __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, double& local_min, int& nan_count) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -1022,7 +1029,9 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
-  const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+  /* strips within three columns of a wall hold ghost lanes; XE fixes that at
+   * compile time (0: none, 1: some), -1 tests it here */
+  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
   const bool last_chunk = (y1 == ny - kPad);
   const int q = x0 - 3 + lane;
   const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
@@ -1230,7 +1239,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   int nan_count = 0;
   for (int item = march_first_uniform(); item < ml.nitems;
        item += gridDim.x * kMWaves) {
-    lagrangian_item<T, P2>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+    if (item_xedge(ml, g.nx, item))
+      lagrangian_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+    else
+      lagrangian_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
   finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);
 }
@@ -1263,7 +1275,7 @@ struct MMom2IO {
  * columns: the y stage's corner flux at face x1 needs u at x1, hence the x
  * stage's cell flux at x1 and u0 up to x1+2). The per-row arithmetic is the
  * two separate kernels', so the results are theirs bit for bit. */
-template <typename T, bool P2>
+template <typename T, bool P2, int XE = -1>
 __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
@@ -1274,7 +1286,9 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
-  const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+  /* strips within three columns of a wall hold ghost lanes; XE fixes that at
+   * compile time (0: none, 1: some), -1 tests it here */
+  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
   const bool last_chunk = (y1 == ny - kPad);
   const int gend = last_chunk ? y1 + 1 : y1;
   const int q = x0 - 2 + lane;
@@ -1454,7 +1468,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
  * flux in x) and y momentum (y flux in y, v recompute, y flux in x) in one
  * march; the y stage at row r reads u of rows r and r-1 from the x stage.
  * Lane l holds x face / column q = x0 - 3 + l (strips of 59 columns). */
-template <typename T, bool P2>
+template <typename T, bool P2, int XE = -1>
 __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
@@ -1465,7 +1479,9 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
-  const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+  /* strips within three columns of a wall hold ghost lanes; XE fixes that at
+   * compile time (0: none, 1: some), -1 tests it here */
+  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
   const bool last_chunk = (y1 == ny - kPad);
   const int gend = last_chunk ? y1 + 1 : y1;
   const int q = x0 - 3 + lane;
@@ -1613,7 +1629,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
   const int lane = threadIdx.x % kWave;
   for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves)
-    momxy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
+    if (item_xedge(ml, g.nx, item))
+      momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+    else
+      momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
 }
 
 template <typename T, bool P2>
@@ -1623,7 +1642,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
   const int lane = threadIdx.x % kWave;
   for (int item = march_first(); item < ml.nitems;
        item += gridDim.x * kMWaves)
-    momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
+    if (item_xedge(ml, g.nx, item))
+      momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+    else
+      momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
 }
 
 /* Persistent grid of a march kernel: as many blocks as stay resident on the
