@@ -52,12 +52,10 @@ static void usage(const char* prog) {
 static int parse_options(int argc, char** argv, Options* o) {
   memset(o, 0, sizeof(*o));
   for (int k = 1; k < argc; ++k) {
-    if (strcmp(argv[k], "--dump") == 0 || strcmp(argv[k], "--load") == 0) {
+    const int dump = strcmp(argv[k], "--dump") == 0, load = strcmp(argv[k], "--load") == 0;
+    if (dump || load) {
       if (k + 1 >= argc) return -1;
-      if (argv[k][2] == 'd')
-        o->dump = argv[++k];
-      else
-        o->load = argv[++k];
+      *(dump ? &o->dump : &o->load) = argv[++k];
     } else if (argv[k][0] == '-' || o->params) {
       return -1;
     } else {
