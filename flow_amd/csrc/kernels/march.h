@@ -60,17 +60,41 @@ struct MarchList {
   int nstrips, nchunks, nitems;
 };
 
-/* A wave's first work item. march_first_uniform makes it wave-uniform:
- * the item, its bounds and the row counter then live in scalar registers,
- * row tests become scalar branches and per-row geometry is read with a
- * scalar lane index. That pays in the Lagrangian and mass/energy marches;
- * in the momentum marches the extra scalar registers it costs made them
- * slower (measured on MI355X), so those keep the per-lane index. */
-__device__ __forceinline__ int march_first() {
-  return blockIdx.x * kMWaves + threadIdx.x / kWave;
-}
-__device__ __forceinline__ int march_first_uniform() {
-  return __builtin_amdgcn_readfirstlane(march_first());
+/* A wave's work items: first, first + stride, ... below end. The item
+ * list is cut into B = min(kMarchBands, gridDim) contiguous bands; band
+ * b = blockIdx % B is walked by the blocks with that residue, kMWaves items
+ * per block at a time. Blocks reach the 8 XCDs round-robin, so with 8 bands
+ * a band is one XCD's share: neighbouring strips and chunks, which read the
+ * same halo cache lines and rows, meet in the same L2. Every item is visited
+ * exactly once for any grid size.
+ *
+ * UNIFORM makes the walk wave-uniform: the item, its bounds and the row
+ * counter then live in scalar registers, row tests become scalar branches
+ * and per-row geometry is read with a scalar lane index. That pays in the
+ * Lagrangian and mass/energy marches; in the momentum marches the extra
+ * scalar registers it costs made them slower (measured on MI355X), so those
+ * keep the per-lane index. */
+#ifndef FLOWHIP_MARCH_BANDS
+#define FLOWHIP_MARCH_BANDS 8
+#endif
+constexpr int kMarchBands = FLOWHIP_MARCH_BANDS;
+
+struct MarchWalk {
+  int first, end, stride;
+};
+
+template <bool UNIFORM>
+__device__ __forceinline__ MarchWalk march_walk(int nitems) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int bands = nb < kMarchBands ? nb : kMarchBands;
+  const int band = b % bands, local = b / bands;
+  const int nb_band = (nb - band + bands - 1) / bands;
+  MarchWalk w;
+  const int first = (int)((long)band * nitems / bands) + local * kMWaves + threadIdx.x / kWave;
+  w.first = UNIFORM ? __builtin_amdgcn_readfirstlane(first) : first;
+  w.end = (int)((long)(band + 1) * nitems / bands);
+  w.stride = nb_band * kMWaves;
+  return w;
 }
 
 inline MarchList make_march(int nx, int ny, int W, int H) {
@@ -353,8 +377,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = march_first_uniform(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<true>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
   finish_sums<kMNT>(ms, es, red);
@@ -489,8 +513,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  for (int item = march_first_uniform(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<true>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
   }
   finish_sums<kMNT>(ms, es, red);
@@ -637,8 +661,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     momx_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
@@ -733,8 +757,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     momx_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
@@ -877,8 +901,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     momy_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
@@ -982,8 +1006,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     momy_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane);
   }
 }
@@ -1237,8 +1261,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   constexpr bool kSq = std::is_same<T, float>::value;
   double local_min = kSq ? max_dt * max_dt : max_dt;
   int nan_count = 0;
-  for (int item = march_first_uniform(); item < ml.nitems;
-       item += gridDim.x * kMWaves) {
+  const MarchWalk mw = march_walk<true>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride) {
     if (item_xedge(ml, g.nx, item))
       lagrangian_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
     else
@@ -1627,8 +1651,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves)
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride)
     if (item_xedge(ml, g.nx, item))
       momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
@@ -1640,8 +1664,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  for (int item = march_first(); item < ml.nitems;
-       item += gridDim.x * kMWaves)
+  const MarchWalk mw = march_walk<false>(ml.nitems);
+  for (int item = mw.first; item < mw.end; item += mw.stride)
     if (item_xedge(ml, g.nx, item))
       momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
