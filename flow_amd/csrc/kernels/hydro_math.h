@@ -177,18 +177,24 @@ template <typename T>
 __device__ __forceinline__ T mc_upwind(T w0, T w1, T w2, T w3, T inv, T vel,
                                        bool upwind_low, T cd_lo, T cd_hi,
                                        T dt_h) {
-  /* Both candidates are evaluated and one is selected: the selected value
-   * is the one the reference's branch computes, and the wave stays
-   * convergent with every stencil load issued up front. */
-  const T a0 = T(0.5) * inv * (w2 - w0);
-  const T b0 = T(2.0) * inv * (w1 - w0);
-  const T c0 = T(2.0) * inv * (w2 - w1);
-  const T lo = w1 + T(0.5) * minmod(minmod(a0, b0), c0) * (cd_lo - vel * dt_h);
-  const T a1 = T(0.5) * inv * (w3 - w1);
-  const T b1 = T(2.0) * inv * (w2 - w1);
-  const T c1 = T(2.0) * inv * (w3 - w2);
-  const T hi = w2 - T(0.5) * minmod(minmod(a1, b1), c1) * (cd_hi + vel * dt_h);
-  return upwind_low ? lo : hi;
+  /* The reference's branch computes one of two mirror-image candidates,
+   *   low  = w1 + 0.5 m(w0, w1, w2) (cd_lo - vel dt_h)
+   *   high = w2 - 0.5 m(w1, w2, w3) (cd_hi + vel dt_h),
+   *   m(p0, p1, p2) = minmod(minmod(0.5 inv (p2 - p0), 2 inv (p1 - p0)),
+   *                          2 inv (p2 - p1)).
+   * The three samples are selected first and one candidate is evaluated, so
+   * the wave stays convergent at half the arithmetic. high is formed as
+   * w2 + (0.5 m) * -(cd_hi + vel dt_h): x - y is x + (-y) in IEEE
+   * arithmetic and a rounded product only changes sign with an operand, so
+   * both candidates round exactly as the reference's expressions. */
+  const T p0 = upwind_low ? w0 : w1;
+  const T p1 = upwind_low ? w1 : w2;
+  const T p2 = upwind_low ? w2 : w3;
+  const T a = T(0.5) * inv * (p2 - p0);
+  const T b = T(2.0) * inv * (p1 - p0);
+  const T c = T(2.0) * inv * (p2 - p1);
+  const T span = upwind_low ? (cd_lo - vel * dt_h) : -(cd_hi + vel * dt_h);
+  return p1 + T(0.5) * minmod(minmod(a, b), c) * span;
 }
 
 template <typename T>
