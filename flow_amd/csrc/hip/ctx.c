@@ -45,11 +45,16 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   HIPCHECK(hipMalloc((void**)&h->red.dtp, sizeof(double) * 2));
   HIPCHECK(hipMalloc((void**)&h->red.sums, sizeof(double) * 4));
   HIPCHECK(hipMalloc((void**)&h->red.status, sizeof(int) * 4));
+  HIPCHECK(hipMalloc((void**)&h->red.queue, sizeof(unsigned int) * FK_MARCH_QUEUE_WORDS));
+  HIPCHECK(hipMalloc((void**)&h->red.totals,
+                     sizeof(unsigned long long) * FK_TOTALS_WORDS));
   HIPCHECK(hipMemset(h->red.partials, 0, sizeof(double) * np));
   HIPCHECK(hipMemset(h->red.ticket, 0, sizeof(unsigned int) * 4));
   HIPCHECK(hipMemset(h->red.dtp, 0, sizeof(double) * 2));
   HIPCHECK(hipMemset(h->red.sums, 0, sizeof(double) * 4));
   HIPCHECK(hipMemset(h->red.status, 0, sizeof(int) * 4));
+  HIPCHECK(hipMemset(h->red.queue, 0, sizeof(unsigned int) * FK_MARCH_QUEUE_WORDS));
+  HIPCHECK(hipMemset(h->red.totals, 0, sizeof(unsigned long long) * FK_TOTALS_WORDS));
   h->red.max_blocks = np / 4;
   HIPCHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 8,
                          hipHostMallocDefault));
@@ -89,6 +94,8 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   hipFree(h->red.dtp);
   hipFree(h->red.sums);
   hipFree(h->red.status);
+  hipFree(h->red.queue);
+  hipFree(h->red.totals);
   hipHostFree(h->h_stage);
   hipHostFree(h->h_status);
   hipStreamDestroy(h->stream);

@@ -101,7 +101,7 @@ static void step_f32(FlowHipCtx* h, Mesh* mesh, FkMesh32 m, int tt, float** a) {
                                      s));
   FK_LAUNCH(h, "advect_momentum_f32",
             fk_march_momentum_f32(m, tt, uF_x, uF_y, F_x, F_y, rho, Qxx, Qyy,
-                                  mom_x, mom_y, u, v, h->red.dtp,
+                                  mom_x, mom_y, u, v, h->red,
                                   flowhip_rows(h, FLOWHIP_K_MOMENTUM, 1), s));
 }
 

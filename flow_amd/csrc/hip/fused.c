@@ -44,7 +44,7 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
     FK_LAUNCH(h, "advect_momentum",
               fk_march_momentum(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho, mx1,
                                 my1, st->mom_x, st->mom_y, st->u, st->v,
-                                h->red.dtp, flowhip_rows(h, FLOWHIP_K_MOMENTUM, 0), s));
+                                h->red, flowhip_rows(h, FLOWHIP_K_MOMENTUM, 0), s));
   } else {
     FK_LAUNCH(h, "advect_momentum_x",
               fk_advect_momentum_x(m, st->tt, u0, v0, st->F_x, st->F_y, st->rho,

@@ -97,6 +97,50 @@ __device__ __forceinline__ MarchWalk march_walk(int nitems) {
   return w;
 }
 
+/* Dynamic form of the banded walk: the waves of band b claim its items one
+ * at a time from band b's counter (atomicAdd by lane 0) as they finish the
+ * previous one, so a wave that draws short or fast items takes more of them
+ * and the band's waves finish together. Claiming the next item when
+ * starting the current one, to hide the claim's round trip, was slower on
+ * MI355X: a wave then holds an item it cannot start, and the band's tail
+ * grows by up to an item. The last wave of the grid to run out (counter
+ * kMarchBands) resets every counter, so they are zero between launches
+ * without a memset (the launches stay capturable in a graph). Every wave
+ * reaches finish(); no wave waits on another. UNIFORM as for march_walk.
+ *
+ * Counter k lives at queue[k * kQueueStride], one per 128-byte line: with
+ * all of them in one line the claims of the eight bands serialise on it,
+ * and the kernels ran 2-3% slower on MI355X. */
+constexpr int kQueueStride = 32;
+static_assert((kMarchBands + 1) * kQueueStride <= FK_MARCH_QUEUE_WORDS, "queue counters");
+
+template <bool UNIFORM>
+struct MarchQueue {
+  unsigned int* ctr;
+  int lo, end, band;
+  __device__ MarchQueue(unsigned int* queue, int nitems) : ctr(queue) {
+    const int nb = gridDim.x;
+    const int bands = nb < kMarchBands ? nb : kMarchBands;
+    band = blockIdx.x % bands;
+    lo = (int)((long)band * nitems / bands);
+    end = (int)((long)(band + 1) * nitems / bands);
+  }
+  /* the next item of the band; end or beyond once the band is drained */
+  __device__ int next() {
+    unsigned int v = 0;
+    if (threadIdx.x % kWave == 0) v = atomicAdd(&ctr[band * kQueueStride], 1u);
+    if (UNIFORM) return lo + (int)__builtin_amdgcn_readfirstlane(v);
+    return lo + (int)__shfl(v, 0, kWave);
+  }
+  __device__ void finish() {
+    if (threadIdx.x % kWave == 0) {
+      const unsigned int waves = gridDim.x * kMWaves;
+      if (atomicAdd(&ctr[kMarchBands * kQueueStride], 1u) == waves - 1)
+        for (int k = 0; k <= kMarchBands; ++k) atomicExch(&ctr[k * kQueueStride], 0u);
+    }
+  }
+};
+
 inline MarchList make_march(int nx, int ny, int W, int H) {
   MarchList m;
   m.W = W;
@@ -376,12 +420,18 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  const MarchWalk mw = march_walk<true>(ml.nitems);
-  for (int item = mw.first; item < mw.end; item += mw.stride) {
+  /* each item's per-lane compensated sums, rounded once, go into exact
+   * accumulators: the totals do not depend on which wave takes which item */
+  FixSum fm{0, 0, 0, 0}, fe{0, 0, 0, 0};
+  MarchQueue<true> mq(red.queue, ml.nitems);
+  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+    CSum ms{0.0, 0.0}, es{0.0, 0.0};
     mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    fm.add(ms.s + ms.c);
+    fe.add(es.s + es.c);
   }
-  finish_sums<kMNT>(ms, es, red);
+  mq.finish();
+  finish_fixed_sums<kMNT>(fm, fe, red);
 }
 
 /* ======================================================================= */
@@ -512,12 +562,18 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
                                                         MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  CSum ms{0.0, 0.0}, es{0.0, 0.0};
-  const MarchWalk mw = march_walk<true>(ml.nitems);
-  for (int item = mw.first; item < mw.end; item += mw.stride) {
+  /* each item's per-lane compensated sums, rounded once, go into exact
+   * accumulators: the totals do not depend on which wave takes which item */
+  FixSum fm{0, 0, 0, 0}, fe{0, 0, 0, 0};
+  MarchQueue<true> mq(red.queue, ml.nitems);
+  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+    CSum ms{0.0, 0.0}, es{0.0, 0.0};
     mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    fm.add(ms.s + ms.c);
+    fe.add(es.s + es.c);
   }
-  finish_sums<kMNT>(ms, es, red);
+  mq.finish();
+  finish_fixed_sums<kMNT>(fm, fe, red);
 }
 
 
@@ -1261,13 +1317,14 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   constexpr bool kSq = std::is_same<T, float>::value;
   double local_min = kSq ? max_dt * max_dt : max_dt;
   int nan_count = 0;
-  const MarchWalk mw = march_walk<true>(ml.nitems);
-  for (int item = mw.first; item < mw.end; item += mw.stride) {
+  MarchQueue<true> mq(red.queue, ml.nitems);
+  for (int item = mq.next(); item < mq.end; item = mq.next()) {
     if (item_xedge(ml, g.nx, item))
       lagrangian_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
     else
       lagrangian_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
+  mq.finish();
   finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);
 }
 
@@ -1646,30 +1703,58 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   }
 }
 
+/* Whether the fused momentum march claims its items from the queue: in
+ * FP64 it ran faster that way; the FP32 kernel ran 3% slower, so it keeps
+ * the static banded walk (measured on MI355X at 8192^2). */
+template <typename T>
+constexpr bool kMomQueue = std::is_same<T, double>::value;
+
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
-    MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
+    MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  const MarchWalk mw = march_walk<false>(ml.nitems);
-  for (int item = mw.first; item < mw.end; item += mw.stride)
-    if (item_xedge(ml, g.nx, item))
-      momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-    else
-      momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  if constexpr (kMomQueue<T>) {
+    MarchQueue<false> mq(queue, ml.nitems);
+    for (int item = mq.next(); item < mq.end; item = mq.next()) {
+      if (item_xedge(ml, g.nx, item))
+        momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      else
+        momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+    }
+    mq.finish();
+  } else {
+    const MarchWalk mw = march_walk<false>(ml.nitems);
+    for (int item = mw.first; item < mw.end; item += mw.stride)
+      if (item_xedge(ml, g.nx, item))
+        momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      else
+        momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  }
 }
 
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
-    MGeo<T> g, MMom2IO<T> io, const double* dtp, MarchList ml) {
+    MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  const MarchWalk mw = march_walk<false>(ml.nitems);
-  for (int item = mw.first; item < mw.end; item += mw.stride)
-    if (item_xedge(ml, g.nx, item))
-      momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-    else
-      momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  if constexpr (kMomQueue<T>) {
+    MarchQueue<false> mq(queue, ml.nitems);
+    for (int item = mq.next(); item < mq.end; item = mq.next()) {
+      if (item_xedge(ml, g.nx, item))
+        momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      else
+        momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+    }
+    mq.finish();
+  } else {
+    const MarchWalk mw = march_walk<false>(ml.nitems);
+    for (int item = mw.first; item < mw.end; item += mw.stride)
+      if (item_xedge(ml, g.nx, item))
+        momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      else
+        momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  }
 }
 
 /* Persistent grid of a march kernel: as many blocks as stay resident on the

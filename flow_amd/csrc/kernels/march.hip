@@ -21,7 +21,7 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kMassW, chunk_rows);
-  if (ml.nitems < 1) return hipErrorInvalidValue;
+  if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
     const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems, mass_blocks_per_cu<T>());
@@ -41,7 +41,7 @@ hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double
                                      int first_step, int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
-  if (ml.nitems < 1) return hipErrorInvalidValue;
+  if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
   const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems);
   hipLaunchKernelGGL((k_march_lagrangian<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
@@ -84,22 +84,24 @@ hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
 /* both momentum directions in one march: strips of 59 columns */
 constexpr int kMomXYW = kWave - 5;
 
+/* queue: the work-queue counters when the kernel claims its items
+ * (kMomQueue<T>), else unused */
 template <typename T, bool P2>
 hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
-                                int chunk_rows, hipStream_t s) {
+                                unsigned int* queue, int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kMomXYW, chunk_rows);
-  if (ml.nitems < 1) return hipErrorInvalidValue;
+  if (ml.nitems < 1 || (kMomQueue<T> && !queue)) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
     const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems);
     hipLaunchKernelGGL((k_march_momxy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       dtp, ml);
+                       dtp, queue, ml);
   } else {
     static int per_cu = 0;
     const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems);
     hipLaunchKernelGGL((k_march_momxy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       dtp, ml);
+                       dtp, queue, ml);
   }
   return hipGetLastError();
 }
@@ -133,9 +135,10 @@ hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
 
 template <typename T>
 hipError_t launch_march_momxy(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
-                              int chunk_rows, int pow2, hipStream_t s) {
-  return pow2 ? launch_march_momxy_p<T, true>(g, io, tt, dtp, chunk_rows, s)
-              : launch_march_momxy_p<T, false>(g, io, tt, dtp, chunk_rows, s);
+                              unsigned int* queue, int chunk_rows, int pow2,
+                              hipStream_t s) {
+  return pow2 ? launch_march_momxy_p<T, true>(g, io, tt, dtp, queue, chunk_rows, s)
+              : launch_march_momxy_p<T, false>(g, io, tt, dtp, queue, chunk_rows, s);
 }
 
 }  // namespace flowhip
@@ -163,10 +166,11 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
 hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
                              const double* F_x, const double* F_y, const double* rho,
                              const double* mx1, const double* my1, double* mom_x,
-                             double* mom_y, double* u, double* v, const double* dtp,
+                             double* mom_y, double* u, double* v, FkReduce r,
                              int chunk_rows, hipStream_t s) {
   MMom2IO<double> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
-  return launch_march_momxy<double>(mgeo64(m), io, tt, dtp, chunk_rows, m.pow2, s);
+  return launch_march_momxy<double>(mgeo64(m), io, tt, r.dtp, r.queue, chunk_rows,
+                                    m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
@@ -225,10 +229,11 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
 hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,
                                  const float* F_x, const float* F_y, const float* rho,
                                  const float* mx1, const float* my1, float* mom_x,
-                                 float* mom_y, float* u, float* v, const double* dtp,
+                                 float* mom_y, float* u, float* v, FkReduce r,
                                  int chunk_rows, hipStream_t s) {
   MMom2IO<float> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
-  return launch_march_momxy<float>(mgeo32(m), io, tt, dtp, chunk_rows, m.pow2, s);
+  return launch_march_momxy<float>(mgeo32(m), io, tt, r.dtp, r.queue, chunk_rows,
+                                   m.pow2, s);
 }
 
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,

@@ -119,4 +119,117 @@ __device__ void finish_sums(CSum ms, CSum es, FkReduce r) {
   }
 }
 
+/* Exact accumulator for totals: a sum of doubles, each first truncated
+ * toward zero to a multiple of 2^-64, held as a 192-bit two's complement
+ * integer in units of 2^-64 (w0 the low word). Integer addition is
+ * associative, so the sum is the same bits whatever the order or grouping
+ * of the additions. Values of magnitude below 2^127 are taken exactly down
+ * to 2^-64; a larger or non-finite value marks the sum bad. */
+struct FixSum {
+  unsigned long long w0, w1, w2;
+  unsigned int bad;
+
+  __device__ void add(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned int E = (unsigned int)(u >> 52) & 0x7ffu;
+    if (E >= 1023u + 127u) { /* |v| >= 2^127, inf or NaN */
+      bad = 1u;
+      return;
+    }
+    const unsigned long long mant = u & 0xfffffffffffffull;
+    const unsigned long long M = E ? (mant | (1ull << 52)) : mant;
+    /* v = M 2^(s - 64) */
+    const int s = (E ? (int)E : 1) - 1075 + 64;
+    unsigned long long a0 = 0, a1 = 0, a2 = 0;
+    if (s < 0) {
+      a0 = s > -64 ? (M >> -s) : 0ull;
+    } else if (s < 64) {
+      a0 = M << s;
+      a1 = s ? (M >> (64 - s)) : 0ull;
+    } else if (s < 128) {
+      const int t = s - 64;
+      a1 = M << t;
+      a2 = t ? (M >> (64 - t)) : 0ull;
+    } else {
+      a2 = M << (s - 128); /* s <= 138: below 2^63 */
+    }
+    if (u >> 63) {
+      const unsigned long long b0 = w0 < a0;
+      w0 -= a0;
+      const unsigned long long t1 = w1 - a1, b1 = (w1 < a1) + (t1 < b0);
+      w1 = t1 - b0;
+      w2 -= a2 + b1;
+    } else {
+      w0 += a0;
+      const unsigned long long c0 = w0 < a0;
+      const unsigned long long t1 = w1 + a1, c1a = t1 < a1;
+      w1 = t1 + c0;
+      w2 += a2 + c1a + (w1 < c0);
+    }
+  }
+};
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+
+/* The double nearest (to within an ulp; the same bits every time) to the
+ * fixed-point total held in 6 limb counters t[0..5] (t[k] counts units of
+ * 2^(32k - 64), any size below 2^63) and its bad count t[6]. */
+__device__ inline double fixed_total(const unsigned long long* t) {
+  if (t[6]) return __builtin_nan("");
+  unsigned int d[6];
+  unsigned long long c = 0;
+  for (int k = 0; k < 6; ++k) {
+    const unsigned long long x = t[k] + c;
+    d[k] = (unsigned int)x;
+    c = x >> 32;
+  }
+  const bool neg = (d[5] >> 31) != 0;
+  if (neg) { /* two's complement negation of the 192-bit value */
+    unsigned long long carry = 1;
+    for (int k = 0; k < 6; ++k) {
+      const unsigned long long x = (unsigned long long)(~d[k]) + carry;
+      d[k] = (unsigned int)x;
+      carry = x >> 32;
+    }
+  }
+  /* exact parts, largest first, into a compensated sum */
+  CSum acc{0.0, 0.0};
+  for (int k = 5; k >= 0; --k) acc.add(__builtin_ldexp((double)d[k], 32 * k - 64));
+  const double r = acc.s + acc.c;
+  return neg ? -r : r;
+}
+
+/* Per-lane exact totals (rho in fm, rho * e in fe) -> r.totals -> last
+ * block -> r.sums[0..1]. Each wave adds its lanes' 32-bit limbs with 64-bit
+ * atomics; the last block reads and clears the counters. The totals are the
+ * same bits whichever waves and blocks added which values. */
+template <int NT>
+__device__ void finish_fixed_sums(const FixSum& fm, const FixSum& fe, FkReduce r) {
+  const FixSum* f[2] = {&fm, &fe};
+  for (int q = 0; q < 2; ++q) {
+    const unsigned long long limb[7] = {
+        f[q]->w0 & 0xffffffffull, f[q]->w0 >> 32, f[q]->w1 & 0xffffffffull,
+        f[q]->w1 >> 32,           f[q]->w2 & 0xffffffffull, f[q]->w2 >> 32,
+        (unsigned long long)f[q]->bad};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const unsigned long long v = wave_sum_u64(limb[k]);
+      if (threadIdx.x % kWave == 0 && v) atomicAdd(&r.totals[8 * q + k], v);
+    }
+  }
+  if (!last_block_arrives(r.ticket)) return;
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 2; ++q) {
+      unsigned long long t[7];
+      for (int k = 0; k < 7; ++k) t[k] = atomicExch(&r.totals[8 * q + k], 0ull);
+      r.sums[q] = fixed_total(t);
+    }
+    *r.ticket = 0u;
+  }
+}
+
 }  // namespace flowhip

@@ -50,7 +50,17 @@ typedef struct FkReduce {
   int* status;           /* [0] NaN CFL candidates being counted, [1] the
                             count of the last fk_timestep launch */
   int max_blocks;
+  /* FK_MARCH_QUEUE_WORDS words holding the counters from which the waves
+   * of a march kernel claim work items; zero between launches (the last
+   * wave resets them). */
+  unsigned int* queue;
+  /* FK_TOTALS_WORDS exact fixed-point accumulators of the mass/energy
+   * march's totals; zero between launches (the last block clears them). */
+  unsigned long long* totals;
 } FkReduce;
+
+#define FK_MARCH_QUEUE_WORDS 512
+#define FK_TOTALS_WORDS 16
 
 /* Number of doubles FkReduce.partials must hold. */
 int fk_reduce_blocks(void);
@@ -189,7 +199,7 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
 hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
                              const double* F_x, const double* F_y, const double* rho,
                              const double* mx1, const double* my1, double* mom_x,
-                             double* mom_y, double* u, double* v, const double* dtp,
+                             double* mom_y, double* u, double* v, FkReduce r,
                              int chunk_rows, hipStream_t s);
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
                                const double* v0, const double* F_x,
@@ -225,7 +235,7 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
 hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,
                                  const float* F_x, const float* F_y, const float* rho,
                                  const float* mx1, const float* my1, float* mom_x,
-                                 float* mom_y, float* u, float* v, const double* dtp,
+                                 float* mom_y, float* u, float* v, FkReduce r,
                                  int chunk_rows, hipStream_t s);
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,
                                    const float* v0, const float* F_x,

@@ -1,7 +1,8 @@
 """flow.hip, the kernel set's own driver (flow's main.c counterpart), on the
 GPU: flow.params' known answer through the same files and printed lines as
 main.c, a non-zero exit when validation fails, --dump equal to the oracle's
-state bit for bit, and --load continuing a run bit for bit."""
+state bit for bit, --load continuing a run bit for bit, and the VisIt
+density snapshots of visit_dump."""
 import json
 import os
 import subprocess
@@ -20,7 +21,7 @@ DRIVER = os.path.join(os.path.dirname(HERE), "flow_amd", "lib", "flow.hip")
 KAT = json.load(open(os.path.join(HERE, "golden", "flow_tests_kat.json")))
 
 
-def _case(tmp_path, n, iterations, dt, boxes, tests=None):
+def _case(tmp_path, n, iterations, dt, boxes, tests=None, visit_dump=0):
     """A run directory with flow.params (and flow.tests) below arch.params."""
     (tmp_path / "arch.params").write_text(
         "width 1.0\nheight 1.0\nmax_dt 1.0e30\nsim_end 1.0e30\n")
@@ -29,7 +30,7 @@ def _case(tmp_path, n, iterations, dt, boxes, tests=None):
     lines = [f"problem_{k} density={b[0]!r} energy={b[1]!r} xpos={b[2]!r} "
              f"ypos={b[3]!r} width={b[4]!r} height={b[5]!r}" for k, b in enumerate(boxes)]
     lines += [f"dt {dt!r}", f"nx {n}", f"ny {n}", f"iterations {iterations}",
-              "visit_dump 0"]
+              f"visit_dump {visit_dump}"]
     (run / "flow.params").write_text("\n".join(lines) + "\n")
     if tests:
         (run / "flow.tests").write_text(tests)
@@ -120,3 +121,30 @@ def test_driver_runs_a_generated_initial_state(tmp_path):
     reg = harness.state_regions(L)
     for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
         assert np.count_nonzero(got["arrays"][k][reg[k]] != v[k][reg[k]]) == 0, k
+
+
+def test_driver_visit_dumps(tmp_path):
+    """visit_dump 1: main.c:108-114 writes density<tt>.{dat,bov} after every
+    iteration (tt from 0) and main.c:129-135 the final density again as
+    density0. The .dat holds the interior densities row by row."""
+    n, steps, dt = 32, 3, 1e-4
+    print(f"case flow.hip visit_dump n={n} steps={steps}", flush=True)
+    run = _case(tmp_path, n, steps, dt, KAT["problems"], visit_dump=1)
+    rc, out, err = _drive(run, "--dump", "final.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    final = sf.read(run / "final.state")
+    L = n + 4
+    interior = np.ascontiguousarray(final["arrays"]["rho"][2:L - 2, 2:L - 2])
+    for tt in (0, steps - 1):
+        got = np.fromfile(run / f"density{tt}.dat", dtype="<f8").reshape(n, n)
+        assert got.tobytes() == interior.tobytes(), tt
+    ref = harness.oracle_run(problems.flow_params_problem, n, 2, dt=dt)
+    mid = harness.views(ref["arrays"], L)["rho"][2:L - 2, 2:L - 2]
+    got = np.fromfile(run / "density1.dat", dtype="<f8").reshape(n, n)
+    assert np.count_nonzero(got != mid) == 0
+    bov = dict(line.split(": ", 1) for line in
+               (run / f"density{steps - 1}.bov").read_text().splitlines())
+    assert bov["DATA_FILE"] == f"density{steps - 1}.dat"
+    assert bov["DATA_SIZE"] == f"{n} {n} 1" and bov["DATA_FORMAT"] == "DOUBLE"
+    assert bov["VARIABLE"] == "density" and bov["CENTERING"] == "zonal"
+    assert bov["TIME"] == f"{final['sim_time']:.12e}"
