@@ -107,12 +107,11 @@ void flowhip_invalidate_sums(Mesh* mesh);
  * FLOWHIP_MARCH_ROWS) applies to all of them; by default each kernel uses
  * the height measured fastest for it on MI355X at 8192^2. Short chunks cost
  * their prologue rows again; with items claimed from a queue the waves end
- * within an item of each other, so shorter items shorten the tail. The
- * FP32 momentum march walks its band statically and prefers long chunks.
- * Every height gives bitwise the same result. */
+ * within an item of each other, so shorter items shorten the tail. Every
+ * height gives bitwise the same result. */
 enum { FLOWHIP_K_LAGRANGIAN = 0, FLOWHIP_K_MASS = 1, FLOWHIP_K_MOMENTUM = 2 };
 static inline int flowhip_rows(const FlowHipCtx* h, int kernel, int fp32) {
-  static const int kRows[2][3] = {{28, 24, 32}, {28, 28, 64}};
+  static const int kRows[2][3] = {{28, 24, 32}, {28, 28, 32}};
   return h->march_rows > 0 ? h->march_rows : kRows[fp32 ? 1 : 0][kernel];
 }
 

@@ -1703,34 +1703,19 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   }
 }
 
-/* Whether the fused momentum march claims its items from the queue: in
- * FP64 it ran faster that way; the FP32 kernel ran 3% slower, so it keeps
- * the static banded walk (measured on MI355X at 8192^2). */
-template <typename T>
-constexpr bool kMomQueue = std::is_same<T, double>::value;
-
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  if constexpr (kMomQueue<T>) {
-    MarchQueue<false> mq(queue, ml.nitems);
-    for (int item = mq.next(); item < mq.end; item = mq.next()) {
-      if (item_xedge(ml, g.nx, item))
-        momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-      else
-        momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
-    }
-    mq.finish();
-  } else {
-    const MarchWalk mw = march_walk<false>(ml.nitems);
-    for (int item = mw.first; item < mw.end; item += mw.stride)
-      if (item_xedge(ml, g.nx, item))
-        momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-      else
-        momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  MarchQueue<false> mq(queue, ml.nitems);
+  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+    if (item_xedge(ml, g.nx, item))
+      momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+    else
+      momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
   }
+  mq.finish();
 }
 
 template <typename T, bool P2>
@@ -1738,30 +1723,21 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  if constexpr (kMomQueue<T>) {
-    MarchQueue<false> mq(queue, ml.nitems);
-    for (int item = mq.next(); item < mq.end; item = mq.next()) {
-      if (item_xedge(ml, g.nx, item))
-        momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-      else
-        momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
-    }
-    mq.finish();
-  } else {
-    const MarchWalk mw = march_walk<false>(ml.nitems);
-    for (int item = mw.first; item < mw.end; item += mw.stride)
-      if (item_xedge(ml, g.nx, item))
-        momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
-      else
-        momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+  MarchQueue<false> mq(queue, ml.nitems);
+  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+    if (item_xedge(ml, g.nx, item))
+      momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+    else
+      momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
   }
+  mq.finish();
 }
 
 /* Persistent grid of a march kernel: as many blocks as stay resident on the
- * device (per_cu per CU, from the occupancy query on first use, at most
- * max_per_cu when that is positive), and no more than the items need. */
+ * device (per_cu per CU, from the occupancy query on first use), and no
+ * more than the items need. */
 template <typename K>
-int march_grid(K kernel, int& per_cu, int nitems, int max_per_cu = 0) {
+int march_grid(K kernel, int& per_cu, int nitems) {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -1777,7 +1753,6 @@ int march_grid(K kernel, int& per_cu, int nitems, int max_per_cu = 0) {
            hipSuccess ||
        per_cu < 1))
     per_cu = 1;
-  if (max_per_cu > 0 && per_cu > max_per_cu) per_cu = max_per_cu;
   long gsz = (long)cus * per_cu;
   const long need = (nitems + kMWaves - 1) / kMWaves;
   if (gsz > need) gsz = need;

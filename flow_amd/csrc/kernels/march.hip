@@ -8,14 +8,6 @@ namespace flowhip {
 
 /* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
 constexpr int kMassW = kWave - 4;
-/* Resident blocks per CU of the mass/energy march (0: as many as fit). The
- * FP32 kernels fit five waves per SIMD; four blocks per CU ran them 3%
- * faster on MI355X at 8192^2 (fewer concurrent rows competing for L2). */
-template <typename T>
-constexpr int mass_blocks_per_cu() {
-  return std::is_same<T, float>::value ? 4 : 0;
-}
-
 template <typename T, bool P2>
 hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
@@ -24,12 +16,12 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems, mass_blocks_per_cu<T>());
+    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems);
     hipLaunchKernelGGL((k_march_mass_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems, mass_blocks_per_cu<T>());
+    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems);
     hipLaunchKernelGGL((k_march_mass_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   }
@@ -84,14 +76,12 @@ hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
 /* both momentum directions in one march: strips of 59 columns */
 constexpr int kMomXYW = kWave - 5;
 
-/* queue: the work-queue counters when the kernel claims its items
- * (kMomQueue<T>), else unused */
 template <typename T, bool P2>
 hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
                                 unsigned int* queue, int chunk_rows, hipStream_t s) {
   if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const MarchList ml = make_march(g.nx, g.ny, kMomXYW, chunk_rows);
-  if (ml.nitems < 1 || (kMomQueue<T> && !queue)) return hipErrorInvalidValue;
+  if (ml.nitems < 1 || !queue) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
     const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems);
