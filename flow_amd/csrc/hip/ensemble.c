@@ -42,6 +42,7 @@ struct FlowHipEnsemble {
   int chunk_rows, order;
   int pow2; /* every instance's spacings are powers of two */
   int fused_momentum;
+  unsigned int* d_queue; /* work-queue counters of the ensemble kernels */
   FkEnsPartials part;
   int part_items;
   FkEnsHistory hist;
@@ -145,6 +146,7 @@ static FkEnsLaunch ens_launch(const FlowHipEnsemble* E) {
   L.chunk_rows = E->chunk_rows;
   L.pow2 = E->pow2;
   L.fused_momentum = E->fused_momentum;
+  L.queue = E->d_queue;
   L.part = E->part;
   L.hist = E->hist;
   return L;
@@ -268,6 +270,8 @@ FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
   HIPCHECK(hipMalloc((void**)&E->d_insts, sizeof(FkEnsInst) * (size_t)ninst));
   HIPCHECK(hipMalloc((void**)&E->hist.counter, sizeof(int)));
   HIPCHECK(hipMemset(E->hist.counter, 0, sizeof(int)));
+  HIPCHECK(hipMalloc((void**)&E->d_queue, sizeof(unsigned int) * FK_MARCH_QUEUE_WORDS));
+  HIPCHECK(hipMemset(E->d_queue, 0, sizeof(unsigned int) * FK_MARCH_QUEUE_WORDS));
   ens_build_work(E);
   return E;
 }
@@ -290,6 +294,7 @@ void flowhip_ensemble_delete(FlowHipEnsemble* E) {
   HIPCHECK(hipFree(E->hist.sums));
   HIPCHECK(hipFree(E->hist.nan));
   HIPCHECK(hipFree(E->hist.counter));
+  HIPCHECK(hipFree(E->d_queue));
   HIPCHECK(hipStreamDestroy(E->stream));
   free(E->gnx);
   free(E->gny);

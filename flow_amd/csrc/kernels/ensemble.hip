@@ -61,12 +61,6 @@ __device__ __forceinline__ MarchList ens_list(const FkEnsInst& in, int H) {
   return ml;
 }
 
-/* The wave's next work-list position, made wave-uniform so that the item
- * and descriptor loads are scalar. */
-__device__ __forceinline__ int ens_first() {
-  return __builtin_amdgcn_readfirstlane(blockIdx.x * kMWaves + threadIdx.x / kWave);
-}
-
 __device__ __forceinline__ int wave_sum_int(int x) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
@@ -78,9 +72,11 @@ __device__ __forceinline__ int wave_sum_int(int x) {
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    unsigned int* __restrict__ queue,
     int H, FkEnsPartials part) {
   const int lane = threadIdx.x % kWave;
-  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+  MarchQueue<true> mq(queue, nwork);
+  for (int w = mq.next(); w < mq.end; w = mq.next()) {
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const MLagIO<T> io{arr<T>(in, kRho),   arr<T>(in, kE),   arr<T>(in, kMomX),
@@ -100,14 +96,17 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
       part.nan[in.item0 + wi.y] = nan_count;
     }
   }
+  mq.finish();
 }
 
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    unsigned int* __restrict__ queue,
     int H, int tt, FkEnsPartials part) {
   const int lane = threadIdx.x % kWave;
-  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+  MarchQueue<true> mq(queue, nwork);
+  for (int w = mq.next(); w < mq.end; w = mq.next()) {
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const MMassIO<T> io{arr<T>(in, kUFx), arr<T>(in, kUFy), arr<T>(in, kRhoOld),
@@ -129,15 +128,18 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
       p[3] = es.c;
     }
   }
+  mq.finish();
 }
 
 /* ydir 0: x momentum (u0 = uF_x), 1: y momentum (io.u = the u just written) */
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    unsigned int* __restrict__ queue,
     int H, int tt, int ydir) {
   const int lane = threadIdx.x % kWave;
-  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+  MarchQueue<true> mq(queue, nwork);
+  for (int w = mq.next(); w < mq.end; w = mq.next()) {
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
@@ -161,15 +163,18 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
         momy_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
     }
   }
+  mq.finish();
 }
 
 /* Both momentum directions in one march (momxy_*_item). */
 template <typename T, bool P2>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    unsigned int* __restrict__ queue,
     int H, int tt) {
   const int lane = threadIdx.x % kWave;
-  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+  MarchQueue<true> mq(queue, nwork);
+  for (int w = mq.next(); w < mq.end; w = mq.next()) {
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
@@ -182,15 +187,18 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
     else
       momxy_yx_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane);
   }
+  mq.finish();
 }
 
 /* CFL candidates of one item's cells (the pre-loop set_timestep). */
 template <typename T>
 __global__ __launch_bounds__(kMNT) void k_ens_timestep(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
+    unsigned int* __restrict__ queue,
     int H, FkEnsPartials part) {
   const int lane = threadIdx.x % kWave;
-  for (int w = ens_first(); w < nwork; w += gridDim.x * kMWaves) {
+  MarchQueue<true> mq(queue, nwork);
+  for (int w = mq.next(); w < mq.end; w = mq.next()) {
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const MGeo<T> g = ens_geo<T>(in);
@@ -222,6 +230,7 @@ __global__ __launch_bounds__(kMNT) void k_ens_timestep(
       part.nan[in.item0 + wi.y] = nan_count;
     }
   }
+  mq.finish();
 }
 
 /* ---- per-instance epilogues: one wave per instance ---- */
@@ -298,28 +307,28 @@ template <typename T, bool P2>
 hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   static int pc_lag = 0, pc_mass = 0, pc_mom = 0;
   const int nw = L.nwork;
-  if (nw < 1 || L.ninst < 1) return hipErrorInvalidValue;
+  if (nw < 1 || L.ninst < 1 || !L.queue) return hipErrorInvalidValue;
   const int2* work = (const int2*)L.work;
   hipLaunchKernelGGL((k_ens_lagrangian<T, P2>), dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw)),
-                     dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, L.part);
+                     dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, tt == 0 ? 1 : 0, std::is_same<T, float>::value ? 1 : 0,
                      L.hist);
   hipLaunchKernelGGL((k_ens_mass<T, P2>), dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw)),
-                     dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt, L.part);
+                     dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, L.hist);
   if (L.fused_momentum) {
     static int pc_momxy = 0;
     hipLaunchKernelGGL((k_ens_momentum_xy<T, P2>),
                        dim3(march_grid(k_ens_momentum_xy<T, P2>, pc_momxy, nw)),
-                       dim3(kMNT), 0, s, L.insts, work, nw, L.chunk_rows, tt);
+                       dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, tt);
   } else {
     const int gm = march_grid(k_ens_momentum<T, P2>, pc_mom, nw);
     hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts,
-                       work, nw, L.chunk_rows, tt, 0);
+                       work, nw, L.queue, L.chunk_rows, tt, 0);
     hipLaunchKernelGGL((k_ens_momentum<T, P2>), dim3(gm), dim3(kMNT), 0, s, L.insts,
-                       work, nw, L.chunk_rows, tt, 1);
+                       work, nw, L.queue, L.chunk_rows, tt, 1);
   }
   hipLaunchKernelGGL(k_ens_advance, dim3(1), dim3(1), 0, s, L.hist.counter);
   return hipGetLastError();
@@ -334,12 +343,12 @@ template <typename T>
 hipError_t ens_timestep(const FkEnsLaunch& L, hipStream_t s) {
   static int pc = 0;
   const int nw = L.nwork;
-  if (nw < 1 || L.ninst < 1) return hipErrorInvalidValue;
+  if (nw < 1 || L.ninst < 1 || !L.queue) return hipErrorInvalidValue;
   FkEnsHistory none = L.hist;
   none.dt = nullptr;
   hipLaunchKernelGGL(k_ens_timestep<T>, dim3(march_grid(k_ens_timestep<T>, pc, nw)),
-                     dim3(kMNT), 0, s, L.insts, (const int2*)L.work, nw, L.chunk_rows,
-                     L.part);
+                     dim3(kMNT), 0, s, L.insts, (const int2*)L.work, nw, L.queue,
+                     L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, 0, 0, none);
   return hipGetLastError();

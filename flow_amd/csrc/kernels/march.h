@@ -114,13 +114,14 @@ __device__ __forceinline__ MarchWalk march_walk(int nitems) {
 constexpr int kQueueStride = 32;
 static_assert((kMarchBands + 1) * kQueueStride <= FK_MARCH_QUEUE_WORDS, "queue counters");
 
-template <bool UNIFORM>
+template <bool UNIFORM, int BANDS = kMarchBands>
 struct MarchQueue {
+  static_assert(BANDS >= 1 && BANDS <= kMarchBands, "bands");
   unsigned int* ctr;
   int lo, end, band;
   __device__ MarchQueue(unsigned int* queue, int nitems) : ctr(queue) {
     const int nb = gridDim.x;
-    const int bands = nb < kMarchBands ? nb : kMarchBands;
+    const int bands = nb < BANDS ? nb : BANDS;
     band = blockIdx.x % bands;
     lo = (int)((long)band * nitems / bands);
     end = (int)((long)(band + 1) * nitems / bands);

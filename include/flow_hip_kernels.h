@@ -282,6 +282,8 @@ typedef struct FkEnsLaunch {
   int ninst, nwork, chunk_rows;
   int pow2;                /* every instance's spacings are powers of two */
   int fused_momentum;      /* both momentum directions in one march */
+  unsigned int* queue;     /* FK_MARCH_QUEUE_WORDS work-queue words, zero
+                              between launches (each kernel resets them) */
   FkEnsPartials part;
   FkEnsHistory hist;
 } FkEnsLaunch;
