@@ -60,24 +60,23 @@ struct MarchList {
   int nstrips, nchunks, nitems;
 };
 
-/* A wave's work items: first, first + stride, ... below end. The item
- * list is cut into B = min(kMarchBands, gridDim) contiguous bands; band
- * b = blockIdx % B is walked by the blocks with that residue, kMWaves items
- * per block at a time. Blocks reach the 8 XCDs round-robin, so with 8 bands
- * a band is one XCD's share: neighbouring strips and chunks, which read the
- * same halo cache lines and rows, meet in the same L2. Every item is visited
- * exactly once for any grid size.
+/* A wave's work items under the static banded walk: first, first + stride,
+ * ... below end. The item list is cut into B = min(kMarchBands, gridDim)
+ * contiguous bands; band b = blockIdx % B is walked by the blocks with that
+ * residue, kMWaves items per block at a time. Blocks reach the 8 XCDs
+ * round-robin, so with 8 bands a band is one XCD's share: neighbouring
+ * strips and chunks, which read the same halo cache lines and rows, meet in
+ * the same L2. Every item is visited exactly once for any grid size. The
+ * split momentum marches walk this way; the other marches claim the items
+ * of the same bands dynamically (MarchQueue below).
  *
- * UNIFORM makes the walk wave-uniform: the item, its bounds and the row
+ * UNIFORM makes the item wave-uniform: the item, its bounds and the row
  * counter then live in scalar registers, row tests become scalar branches
  * and per-row geometry is read with a scalar lane index. That pays in the
  * Lagrangian and mass/energy marches; in the momentum marches the extra
  * scalar registers it costs made them slower (measured on MI355X), so those
  * keep the per-lane index. */
-#ifndef FLOWHIP_MARCH_BANDS
-#define FLOWHIP_MARCH_BANDS 8
-#endif
-constexpr int kMarchBands = FLOWHIP_MARCH_BANDS;
+constexpr int kMarchBands = kXcds;
 
 struct MarchWalk {
   int first, end, stride;

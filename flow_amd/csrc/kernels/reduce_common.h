@@ -1,7 +1,8 @@
 /*
  * reduce_common.h -- block-level reductions and the single-pass epilogues
- * (last block to arrive combines the per-block partials) shared by the
- * reduction kernels and the fused kernels.
+ * (the last block to arrive combines what the others left) shared by the
+ * reduction kernels, the fused kernels and the march kernels: the CFL
+ * minimum, compensated per-block totals, and exact fixed-point totals.
  */
 #pragma once
 
