@@ -8,10 +8,16 @@ from flow_amd import problems
 PAD = 2
 
 
+def sides(n):
+    """(nx, ny) of a mesh given as n (square) or (nx, ny)."""
+    return (n, n) if np.isscalar(n) else (int(n[0]), int(n[1]))
+
+
 def oracle_run(problem_fn, n, steps, width=1.0, max_dt=1.0e30, dt=1.0e-4,
                init=None):
     from oracle import pyoracle as po
-    m = po.Mesh(n, n, width=width, height=width, max_dt=max_dt, dt=dt)
+    nx, ny = sides(n)
+    m = po.Mesh(nx, ny, width=width, height=width, max_dt=max_dt, dt=dt)
     a = po.new_arrays(m)
     rho, e = init if init is not None else problem_fn(n)
     a["rho"][:] = rho
@@ -19,13 +25,14 @@ def oracle_run(problem_fn, n, steps, width=1.0, max_dt=1.0e30, dt=1.0e-4,
     po.reflect_initial(m, a)
     steps_done, dts, mass, energy, wall = po.run(m, a, steps)
     return dict(arrays=a, dt=dts, mass=mass, energy=energy, steps=steps_done,
-                L=m.nx, wall=wall)
+                L=m.nx if nx == ny else (m.nx, m.ny), wall=wall)
 
 
 def gpu_run(problem_fn, n, steps, schedule, width=1.0, max_dt=1.0e30,
             dt=1.0e-4, graphs=False, init=None, march_rows=None):
     import flow_amd
-    m = flow_amd.Mesh(n, n, width=width, height=width, max_dt=max_dt, dt=dt)
+    nx, ny = sides(n)
+    m = flow_amd.Mesh(nx, ny, width=width, height=width, max_dt=max_dt, dt=dt)
     m.set_schedule(schedule)
     if march_rows:
         m.set_march_rows(march_rows)
@@ -37,21 +44,24 @@ def gpu_run(problem_fn, n, steps, schedule, width=1.0, max_dt=1.0e30,
     st.reflect_initial()
     steps_done, dts, mass, energy, wall = st.run(steps)
     out = dict(arrays=st.download_all(), dt=dts, mass=mass, energy=energy,
-               steps=steps_done, L=m.nx, wall=wall)
+               steps=steps_done, L=m.nx if nx == ny else (m.nx, m.ny), wall=wall)
     st.close()
     m.close()
     return out
 
 
 def views(a, L):
-    """2-D views of the flat arrays by layout."""
+    """2-D views of the flat arrays by layout; L is the padded side, or
+    (Lx, Ly). Cell arrays are Ly rows of Lx, x-face arrays Ly rows of Lx+1,
+    y-face arrays Ly+1 rows of Lx (strides local_nx and local_nx + 1)."""
+    Lx, Ly = sides(L)
     return {
-        "rho": a["rho"][: L * L].reshape(L, L),
-        "e": a["e"][: L * L].reshape(L, L),
-        "u": a["u"][: L * (L + 1)].reshape(L, L + 1),
-        "v": a["v"][: (L + 1) * L].reshape(L + 1, L),
-        "mom_x": a["mom_x"][: L * (L + 1)].reshape(L, L + 1),
-        "mom_y": a["mom_y"][: (L + 1) * L].reshape(L + 1, L),
+        "rho": a["rho"][: Lx * Ly].reshape(Ly, Lx),
+        "e": a["e"][: Lx * Ly].reshape(Ly, Lx),
+        "u": a["u"][: Ly * (Lx + 1)].reshape(Ly, Lx + 1),
+        "v": a["v"][: (Ly + 1) * Lx].reshape(Ly + 1, Lx),
+        "mom_x": a["mom_x"][: Ly * (Lx + 1)].reshape(Ly, Lx + 1),
+        "mom_y": a["mom_y"][: (Ly + 1) * Lx].reshape(Ly + 1, Lx),
     }
 
 
@@ -67,14 +77,15 @@ def noncorner_mask(rows, cols, pad=PAD):
 def state_regions(L):
     """Where each carried field is defined after a step: cell fields and
     velocities everywhere but the corners; momenta on their face domains
-    (omp3's FX / FY, omp3/flow.c:578-580, 631-633)."""
-    mx = np.zeros((L, L + 1), dtype=bool)
-    mx[PAD:L - PAD, PAD:L + 1 - PAD] = True
-    my = np.zeros((L + 1, L), dtype=bool)
-    my[PAD:L + 1 - PAD, PAD:L - PAD] = True
+    (omp3's FX / FY, omp3/flow.c:578-580, 631-633). L as for views."""
+    Lx, Ly = sides(L)
+    mx = np.zeros((Ly, Lx + 1), dtype=bool)
+    mx[PAD:Ly - PAD, PAD:Lx + 1 - PAD] = True
+    my = np.zeros((Ly + 1, Lx), dtype=bool)
+    my[PAD:Ly + 1 - PAD, PAD:Lx - PAD] = True
     return {
-        "rho": noncorner_mask(L, L), "e": noncorner_mask(L, L),
-        "u": noncorner_mask(L, L + 1), "v": noncorner_mask(L + 1, L),
+        "rho": noncorner_mask(Ly, Lx), "e": noncorner_mask(Ly, Lx),
+        "u": noncorner_mask(Ly, Lx + 1), "v": noncorner_mask(Ly + 1, Lx),
         "mom_x": mx, "mom_y": my,
     }
 

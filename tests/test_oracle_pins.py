@@ -438,3 +438,23 @@ def test_corner_ghosts_do_not_reach_the_interior(oracle):
         A2 = as2d(a2[k], L, L)[PAD:L - PAD, PAD:L - PAD]
         np.testing.assert_array_equal(A1, A2)
     np.testing.assert_array_equal(o1[1], o2[1])
+
+
+def test_rectangular_sod_y_keeps_columns_identical(oracle):
+    """On an nx != ny mesh (12 x 40), a split along y keeps every column of
+    rho identical and u zero, and conserves mass: a swapped stride or bound
+    between x and y breaks all three."""
+    from tests import harness
+    nx, ny = 12, 40
+    init = problems.boxes(nx, ny, [(0.125, 2.0, 0.0, 0.0, 1.0, 1.0),
+                                   (1.0, 2.5, 0.0, 0.0, 1.0, 0.5)])
+    Lx, Ly = nx + 2 * PAD, ny + 2 * PAD
+    m0 = math.fsum(as2d(init[0], Ly, Lx)[PAD:Ly - PAD, PAD:Lx - PAD].ravel())
+    r = harness.oracle_run(None, (nx, ny), 10, init=init)
+    assert r["L"] == (Lx, Ly)
+    R = as2d(r["arrays"]["rho"], Ly, Lx)[PAD:Ly - PAD, PAD:Lx - PAD]
+    assert np.all(R == R[:, :1])
+    assert np.any(R[:, 0] != R[0, 0])
+    assert np.all(r["arrays"]["u"] == 0.0)
+    assert np.any(as2d(r["arrays"]["v"], Ly + 1, Lx) != 0.0)
+    assert np.max(np.abs(np.asarray(r["mass"]) - m0)) <= 1e-13 * m0
