@@ -251,9 +251,9 @@ def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (the multi-GPU row-slab design is documented, not "
-              "built)", file=sys.stderr)
+        print(f"bench.py: the hip kernel set runs on one GPU; --gpus {args.gpus} "
+              "is not supported (the multi-GPU row-slab design is documented, "
+              "not built)", file=sys.stderr)
         sys.exit(2)
     import flow_amd
     from flow_amd import native, problems

@@ -307,8 +307,9 @@ __global__ void k_mom_y_update(FkMesh m, Domain d, const double* dtp,
   }
 }
 
-/* Loop domains of omp3 (SURVEY notation): interior cells I, x faces FX,
- * y faces FY, and FXY = [pad, L+1-pad)^2. */
+/* Loop domains of omp3: interior cells I = [pad, L-pad)^2, x faces FX
+ * (columns up to L+1-pad), y faces FY (rows up to L+1-pad), and
+ * FXY = [pad, L+1-pad)^2. */
 inline Domain dom_all(FkMesh m) { return Domain{0, m.ny, 0, m.nx}; }
 inline Domain dom_I(FkMesh m) {
   return Domain{m.pad, m.ny - m.pad, m.pad, m.nx - m.pad};

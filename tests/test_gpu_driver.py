@@ -21,10 +21,11 @@ DRIVER = os.path.join(os.path.dirname(HERE), "flow_amd", "lib", "flow.hip")
 KAT = json.load(open(os.path.join(HERE, "golden", "flow_tests_kat.json")))
 
 
-def _case(tmp_path, n, iterations, dt, boxes, tests=None, visit_dump=0):
+def _case(tmp_path, n, iterations, dt, boxes, tests=None, visit_dump=0,
+          sim_end=1.0e30):
     """A run directory with flow.params (and flow.tests) below arch.params."""
     (tmp_path / "arch.params").write_text(
-        "width 1.0\nheight 1.0\nmax_dt 1.0e30\nsim_end 1.0e30\n")
+        f"width 1.0\nheight 1.0\nmax_dt 1.0e30\nsim_end {sim_end!r}\n")
     run = tmp_path / "flow"
     run.mkdir(exist_ok=True)
     lines = [f"problem_{k} density={b[0]!r} energy={b[1]!r} xpos={b[2]!r} "
@@ -148,3 +149,21 @@ def test_driver_visit_dumps(tmp_path):
     assert bov["DATA_SIZE"] == f"{n} {n} 1" and bov["DATA_FORMAT"] == "DOUBLE"
     assert bov["VARIABLE"] == "density" and bov["CENTERING"] == "zonal"
     assert bov["TIME"] == f"{final['sim_time']:.12e}"
+
+
+def test_driver_stops_at_sim_end(tmp_path):
+    """main.c:95-101: once the simulated time reaches sim_end the loop stops
+    after that step's conservation line; the dump records the steps done."""
+    n = 32
+    print(f"case flow.hip sim_end n={n}", flush=True)
+    run = _case(tmp_path, n, 20, 1e-4, KAT["problems"], sim_end=1e-9)
+    rc, out, err = _drive(run, "--dump", "final.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    assert "reached end of simulation time" in out
+    assert out.count("\nIteration ") == 1
+    assert out.count("Total mass:") == 1
+    got = sf.read(run / "final.state")
+    assert got["step"] == 1
+    # main.c:95 adds mesh.dt after solve_hydro_2d: the dt step 1 advected with
+    ref = harness.oracle_run(problems.flow_params_problem, n, 1, dt=1e-4)
+    assert got["sim_time"] == ref["dt"][1][0]

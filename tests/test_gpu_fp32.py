@@ -20,9 +20,9 @@ DT_REL = 1.0e-5
 TOTAL_REL = 1.0e-6
 
 
-def gpu_run_f32(problem_fn, n, steps, dt, init=None):
+def gpu_run_f32(problem_fn, n, steps, dt, init=None, max_dt=1.0e30):
     import flow_amd
-    m = flow_amd.Mesh(n, n, dt=dt)
+    m = flow_amd.Mesh(n, n, dt=dt, max_dt=max_dt)
     st = flow_amd.DeviceState32(m)
     rho, e = init if init is not None else problem_fn(n)
     st.upload("rho", rho)
@@ -104,3 +104,20 @@ def test_fp32_near_vacuum_cells_stay_finite():
         print(f"  {k:4s} rel_linf={linf:.3e} rel_l1={l1:.3e}", flush=True)
         assert linf <= FIELD_REL_LINF, (k, linf)
     assert np.all(np.isfinite(got["arrays"]["e"]))
+
+
+@pytest.mark.gpu
+def test_fp32_state_binding_max_dt():
+    """max_dt below the CFL step: the FP32 form compares squared candidates
+    starting from max_dt^2 and takes one square root of the minimum, so its
+    dt sequence still follows omp3's (omp3/flow.c:80, 103-105)."""
+    n, steps, dt, max_dt = 64, 10, 1e-4, 2e-5
+    print(f"case fp32 max_dt multi_blast n={n} steps={steps} max_dt={max_dt}", flush=True)
+    ref = harness.oracle_run(problems.multi_blast, n, steps, max_dt=max_dt, dt=dt)
+    got = gpu_run_f32(problems.multi_blast, n, steps, dt, max_dt=max_dt)
+    dt_rel = np.max(np.abs(got["dt"] - ref["dt"]) / np.abs(ref["dt"]))
+    print(f"  dt_rel={dt_rel:.3e}", flush=True)
+    assert dt_rel <= DT_REL
+    errs = field_errors(got["arrays"], ref["arrays"], ref["L"])
+    for k, (linf, _) in errs.items():
+        assert linf <= FIELD_REL_LINF, (k, linf)

@@ -1,6 +1,6 @@
 """Parity of the hip kernel set with the oracle on identical inputs.
 
-The FP64 bar (BASELINE north star): rho, e, u and v within relative L-inf
+The FP64 bar: rho, e, u and v within relative L-inf
 1e-11 after N steps, the dt sequence within 1e-12 relative, the mass and
 energy totals to round-off. The kernels keep omp3's expression trees with
 contraction off, so the tests also count bitwise differences, which are
@@ -175,3 +175,20 @@ def test_graph_replay_matches_oracle_bitwise(schedule):
             assert ndiff == 0, (k, rel, ndiff)
         np.testing.assert_allclose(mass, ref["mass"], rtol=1e-15, atol=0)
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["stagewise", "fused", "march"])
+def test_binding_max_dt_matches_oracle_bitwise(schedule):
+    """max_dt below the CFL step (2e-5 on the 64^2 multi-blast, whose CFL
+    step grows to ~1.6e-4): the minimum starts from max_dt
+    (omp3/flow.c:80) and the dt recurrence settles at C_T * max_dt."""
+    sched = {"stagewise": flow_amd.STAGEWISE, "fused": flow_amd.FUSED,
+             "march": flow_amd.MARCH}[schedule]
+    n, steps, dt, max_dt = 64, 10, 1e-4, 2e-5
+    print(f"case max_dt {schedule} multi_blast n={n} steps={steps} max_dt={max_dt}",
+          flush=True)
+    ref = harness.oracle_run(problems.multi_blast, n, steps, max_dt=max_dt, dt=dt)
+    assert ref["dt"][-1][0] == pytest.approx(0.3 * max_dt, rel=1e-12)
+    got = harness.gpu_run(problems.multi_blast, n, steps, sched, max_dt=max_dt, dt=dt)
+    _check(got, ref, ref["L"], bitwise=True)
