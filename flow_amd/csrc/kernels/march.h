@@ -141,6 +141,35 @@ struct MarchQueue {
   }
 };
 
+/* Block form of the queue, for the fused momentum marches: thread 0 claims
+ * kMWaves consecutive items of the band for the whole block and wave w
+ * marches base + w, so a block's waves march adjacent strips of one chunk
+ * row side by side and share the halo columns between them in cache (the
+ * other marches ran slower this way, waiting at the barriers). Every wave
+ * of the block takes part in every claim, and the loop ends for the whole
+ * block at once:
+ *
+ *   MarchBlockQueue mq(queue, nitems);
+ *   for (int base = mq.next(); base < mq.end; base = mq.next()) {
+ *     const int item = base + wave;
+ *     if (item < mq.end) ... march item ...
+ *   }
+ *   mq.finish();
+ */
+struct MarchBlockQueue : MarchQueue<true> {
+  __device__ MarchBlockQueue(unsigned int* queue, int nitems)
+      : MarchQueue<true>(queue, nitems) {}
+  /* the block's next base item; the same in every wave */
+  __device__ int next() {
+    __shared__ int s_base;
+    __syncthreads(); /* every wave has read the previous base */
+    if (threadIdx.x == 0)
+      s_base = lo + (int)atomicAdd(&ctr[band * kQueueStride], (unsigned int)kMWaves);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(s_base);
+  }
+};
+
 inline MarchList make_march(int nx, int ny, int W, int H) {
   MarchList m;
   m.W = W;
@@ -1708,8 +1737,11 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  MarchQueue<false> mq(queue, ml.nitems);
-  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+  const int wave = threadIdx.x / kWave;
+  MarchBlockQueue mq(queue, ml.nitems);
+  for (int base = mq.next(); base < mq.end; base = mq.next()) {
+    const int item = base + wave;
+    if (item >= mq.end) continue;
     if (item_xedge(ml, g.nx, item))
       momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
@@ -1723,8 +1755,11 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
-  MarchQueue<false> mq(queue, ml.nitems);
-  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+  const int wave = threadIdx.x / kWave;
+  MarchBlockQueue mq(queue, ml.nitems);
+  for (int base = mq.next(); base < mq.end; base = mq.next()) {
+    const int item = base + wave;
+    if (item >= mq.end) continue;
     if (item_xedge(ml, g.nx, item))
       momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
