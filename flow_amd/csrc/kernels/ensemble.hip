@@ -173,8 +173,12 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
     unsigned int* __restrict__ queue,
     int H, int tt) {
   const int lane = threadIdx.x % kWave;
-  MarchQueue<true> mq(queue, nwork);
-  for (int w = mq.next(); w < mq.end; w = mq.next()) {
+  /* claimed per block, as the single-mesh fused momentum march */
+  const int wave = threadIdx.x / kWave;
+  MarchBlockQueue mq(queue, nwork);
+  for (int base = mq.next(); base < mq.end; base = mq.next()) {
+    const int w = __builtin_amdgcn_readfirstlane(base + wave);
+    if (w >= mq.end) continue;
     const int2 wi = work[w];
     const FkEnsInst& in = insts[wi.x];
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
