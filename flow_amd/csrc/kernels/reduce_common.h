@@ -176,6 +176,12 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
   return x;
 }
 
+/* Accumulator word k of total q lives at r.totals[(8 q + k) * kTotalsStride],
+ * one per 128-byte line, so the blocks' atomics on different words do not
+ * queue on one line. */
+constexpr int kTotalsStride = 16;
+static_assert(16 * kTotalsStride <= FK_TOTALS_WORDS, "totals words");
+
 /* The double nearest (to within an ulp; the same bits every time) to the
  * fixed-point total held in 6 limb counters t[0..5] (t[k] counts units of
  * 2^(32k - 64), any size below 2^63) and its bad count t[6]. */
@@ -205,12 +211,18 @@ __device__ inline double fixed_total(const unsigned long long* t) {
 }
 
 /* Per-lane exact totals (rho in fm, rho * e in fe) -> r.totals -> last
- * block -> r.sums[0..1]. Each wave adds its lanes' 32-bit limbs with 64-bit
- * atomics; the last block reads and clears the counters. The totals are the
- * same bits whichever waves and blocks added which values. */
+ * block -> r.sums[0..1]. The block sums its lanes' 32-bit limbs (integer
+ * sums, in any order) and its first 14 threads add them to the counters
+ * with 64-bit atomics; the last block reads and clears the counters. The
+ * totals are the same bits whichever waves and blocks added which values.
+ * (One atomic per wave and limb, on counters sharing a line, cost the
+ * mass/energy march 2-3%.) */
 template <int NT>
 __device__ void finish_fixed_sums(const FixSum& fm, const FixSum& fe, FkReduce r) {
+  __shared__ unsigned long long part[NT / kWave][14];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   const FixSum* f[2] = {&fm, &fe};
+#pragma unroll
   for (int q = 0; q < 2; ++q) {
     const unsigned long long limb[7] = {
         f[q]->w0 & 0xffffffffull, f[q]->w0 >> 32, f[q]->w1 & 0xffffffffull,
@@ -219,14 +231,22 @@ __device__ void finish_fixed_sums(const FixSum& fm, const FixSum& fe, FkReduce r
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       const unsigned long long v = wave_sum_u64(limb[k]);
-      if (threadIdx.x % kWave == 0 && v) atomicAdd(&r.totals[8 * q + k], v);
+      if (lane == 0) part[w][7 * q + k] = v;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 14) {
+    unsigned long long v = 0;
+    for (int b = 0; b < NT / kWave; ++b) v += part[b][threadIdx.x];
+    const int q = threadIdx.x / 7, k = threadIdx.x % 7;
+    if (v) atomicAdd(&r.totals[(8 * q + k) * kTotalsStride], v);
   }
   if (!last_block_arrives(r.ticket)) return;
   if (threadIdx.x == 0) {
     for (int q = 0; q < 2; ++q) {
       unsigned long long t[7];
-      for (int k = 0; k < 7; ++k) t[k] = atomicExch(&r.totals[8 * q + k], 0ull);
+      for (int k = 0; k < 7; ++k)
+        t[k] = atomicExch(&r.totals[(8 * q + k) * kTotalsStride], 0ull);
       r.sums[q] = fixed_total(t);
     }
     *r.ticket = 0u;

@@ -60,7 +60,7 @@ typedef struct FkReduce {
 } FkReduce;
 
 #define FK_MARCH_QUEUE_WORDS 512
-#define FK_TOTALS_WORDS 16
+#define FK_TOTALS_WORDS 256
 
 /* Number of doubles FkReduce.partials must hold. */
 int fk_reduce_blocks(void);
