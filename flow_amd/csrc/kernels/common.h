@@ -40,8 +40,8 @@ __device__ __forceinline__ double wave_min(double x) {
 }
 
 /* Error-free transformation a + b = s + err (Knuth TwoSum). */
-__device__ __forceinline__ void two_sum(double a, double b, double& s,
-                                        double& err) {
+__host__ __device__ __forceinline__ void two_sum(double a, double b, double& s,
+                                                 double& err) {
   s = a + b;
   const double bp = s - a;
   const double ap = s - bp;
@@ -51,13 +51,13 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s,
 /* Compensated accumulator (sum, compensation), merged with TwoSum. */
 struct CSum {
   double s, c;
-  __device__ void add(double x) {
+  __host__ __device__ void add(double x) {
     double t, e;
     two_sum(s, x, t, e);
     s = t;
     c += e;
   }
-  __device__ void merge(const CSum& o) {
+  __host__ __device__ void merge(const CSum& o) {
     double t, e;
     two_sum(s, o.s, t, e);
     s = t;

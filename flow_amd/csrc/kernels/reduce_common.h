@@ -130,8 +130,8 @@ struct FixSum {
   unsigned long long w0, w1, w2;
   unsigned int bad;
 
-  __device__ void add(double v) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  __host__ __device__ void add(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
     const unsigned int E = (unsigned int)(u >> 52) & 0x7ffu;
     if (E >= 1023u + 127u) { /* |v| >= 2^127, inf or NaN */
       bad = 1u;
@@ -185,7 +185,7 @@ static_assert(16 * kTotalsStride <= FK_TOTALS_WORDS, "totals words");
 /* The double nearest (to within an ulp; the same bits every time) to the
  * fixed-point total held in 6 limb counters t[0..5] (t[k] counts units of
  * 2^(32k - 64), any size below 2^63) and its bad count t[6]. */
-__device__ inline double fixed_total(const unsigned long long* t) {
+__host__ __device__ inline double fixed_total(const unsigned long long* t) {
   if (t[6]) return __builtin_nan("");
   unsigned int d[6];
   unsigned long long c = 0;
