@@ -54,6 +54,11 @@ constexpr int kMNT = 256;            /* threads per block: 4 independent waves *
 #define FLOWHIP_MARCH_WAVES 1
 #endif
 constexpr int kMWaves = kMNT / kWave;
+/* Resident blocks per CU (= waves per SIMD) of the Lagrangian march: its
+ * FP64 form fits four (126 VGPRs) but runs 1.5% faster with three (measured
+ * on MI355X at 8192^2); the FP32 form keeps what fits. 0: no cap. */
+template <typename T>
+constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 0;
 
 struct MarchList {
   int W, H;           /* strip width (output columns), chunk height (rows) */
@@ -266,9 +271,9 @@ struct RowGeo {
  * geometry arrays are not written while a march kernel runs, so they are
  * read through the constant address space and the compiler issues scalar
  * loads (no vector issue slots, no lane reads). Same values as RowGeo for
- * the same rows, clamped the same way. Measured on MI355X at 8192^2: 2.5%
- * off the FP64 mass/energy march and 2% off the FP32 Lagrangian march, 2.5%
- * slower in the FP64 Lagrangian march, which keeps RowGeo. */
+ * the same rows, clamped the same way. Used where the row is wave-uniform
+ * (the Lagrangian and mass/energy marches); measured on MI355X at 8192^2,
+ * 1-2.5% off each of those kernels. */
 template <typename T>
 struct RowScalar {
   const __attribute__((address_space(4))) T* p;
@@ -1168,12 +1173,10 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   const bool own_face = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
   const bool own = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-  using Row = typename std::conditional<std::is_same<T, float>::value, RowScalar<T>,
-                                        RowGeo<T>>::type;
-  const Row edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T iedx_q = g.iedx[qg], rcdx_q = T(1.0) / cdx_q;
-  const Row iedy(g.iedy, y0 - 4, ny, lane);
+  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* y face k of the lane's column after pressure acceleration, from cell
@@ -1790,10 +1793,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
 }
 
 /* Persistent grid of a march kernel: as many blocks as stay resident on the
- * device (per_cu per CU, from the occupancy query on first use), and no
- * more than the items need. */
+ * device (per_cu per CU, from the occupancy query on first use, at most
+ * max_per_cu when that is positive), and no more than the items need. */
 template <typename K>
-int march_grid(K kernel, int& per_cu, int nitems) {
+int march_grid(K kernel, int& per_cu, int nitems, int max_per_cu = 0) {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -1809,6 +1812,7 @@ int march_grid(K kernel, int& per_cu, int nitems) {
            hipSuccess ||
        per_cu < 1))
     per_cu = 1;
+  if (max_per_cu > 0 && per_cu > max_per_cu) per_cu = max_per_cu;
   long gsz = (long)cus * per_cu;
   const long need = (nitems + kMWaves - 1) / kMWaves;
   if (gsz > need) gsz = need;

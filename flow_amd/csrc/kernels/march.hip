@@ -35,7 +35,7 @@ hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double
   const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
   if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
-  const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems);
+  const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems, kLagMaxWaves<T>);
   hipLaunchKernelGGL((k_march_lagrangian<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                      (const double*)r.dtp, r, max_dt, first_step, ml);
   return hipGetLastError();
