@@ -59,6 +59,11 @@ constexpr int kMWaves = kMNT / kWave;
  * on MI355X at 8192^2); the FP32 form keeps what fits. 0: no cap. */
 template <typename T>
 constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 0;
+/* The same for the fused momentum march: with wave-uniform items it needs
+ * 115-117 VGPRs in FP64 and 66-70 in FP32, room for four and seven waves;
+ * three and five (what fitted before) are 3% and 4% faster. */
+template <typename T>
+constexpr int kMomMaxWaves = sizeof(T) == 8 ? 3 : 5;
 
 struct MarchList {
   int W, H;           /* strip width (output columns), chunk height (rows) */
@@ -272,8 +277,8 @@ struct RowGeo {
  * read through the constant address space and the compiler issues scalar
  * loads (no vector issue slots, no lane reads). Same values as RowGeo for
  * the same rows, clamped the same way. Used where the row is wave-uniform
- * (the Lagrangian and mass/energy marches); measured on MI355X at 8192^2,
- * 1-2.5% off each of those kernels. */
+ * (every fused march; the items are claimed per wave); measured on MI355X
+ * at 8192^2, 1-2.5% off the Lagrangian and mass/energy marches. */
 template <typename T>
 struct RowScalar {
   const __attribute__((address_space(4))) T* p;
@@ -1433,8 +1438,8 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   const bool own_y = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
   const T iedx_q = g.iedx[qg];
-  const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
   const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
@@ -1627,8 +1632,8 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   const bool own_y = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
   const T iedx_q = g.iedx[qg];
-  const RowGeo<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowGeo<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
   const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
@@ -1764,7 +1769,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
   const int wave = threadIdx.x / kWave;
   MarchBlockQueue mq(queue, ml.nitems);
   for (int base = mq.next(); base < mq.end; base = mq.next()) {
-    const int item = base + wave;
+    const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
     if (item_xedge(ml, g.nx, item))
       momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
@@ -1782,7 +1787,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
   const int wave = threadIdx.x / kWave;
   MarchBlockQueue mq(queue, ml.nitems);
   for (int base = mq.next(); base < mq.end; base = mq.next()) {
-    const int item = base + wave;
+    const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
     if (item_xedge(ml, g.nx, item))
       momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);

@@ -84,12 +84,12 @@ hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* 
   if (ml.nitems < 1 || !queue) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems);
+    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems, kMomMaxWaves<T>);
     hipLaunchKernelGGL((k_march_momxy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        dtp, queue, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems);
+    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems, kMomMaxWaves<T>);
     hipLaunchKernelGGL((k_march_momxy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        dtp, queue, ml);
   }
