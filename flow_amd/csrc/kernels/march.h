@@ -54,16 +54,15 @@ constexpr int kMNT = 256;            /* threads per block: 4 independent waves *
 #define FLOWHIP_MARCH_WAVES 1
 #endif
 constexpr int kMWaves = kMNT / kWave;
-/* Resident blocks per CU (= waves per SIMD) of the Lagrangian march: its
- * FP64 form fits four (126 VGPRs) but runs 1.5% faster with three (measured
- * on MI355X at 8192^2); the FP32 form keeps what fits. 0: no cap. */
+/* Resident blocks per CU (= waves per SIMD) of the fused march kernels:
+ * three in FP64, five in FP32. Their registers would let some of them fit
+ * more (FP64 momentum 111-119 VGPRs, Lagrangian 121, mass/energy 126-129:
+ * four; FP32 momentum 66-70: seven), and each extra wave measured slower on
+ * MI355X at 8192^2 (Lagrangian +2.5%, momentum +3% FP64 and +4% FP32, with
+ * six FP32 waves +3%); with four FP32 waves the momentum march was 2.5%
+ * slower. */
 template <typename T>
-constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 0;
-/* The same for the fused momentum march: with wave-uniform items it needs
- * 115-117 VGPRs in FP64 and 66-70 in FP32, room for four and seven waves;
- * three and five (what fitted before) are 3% and 4% faster. */
-template <typename T>
-constexpr int kMomMaxWaves = sizeof(T) == 8 ? 3 : 5;
+constexpr int kMarchMaxWaves = sizeof(T) == 8 ? 3 : 5;
 
 struct MarchList {
   int W, H;           /* strip width (output columns), chunk height (rows) */
@@ -290,6 +289,35 @@ struct RowScalar {
   __device__ __forceinline__ T operator()(int r) const { return p[max(0, min(r, last))]; }
 };
 
+/* 1/x for a positive power of two x with a normal reciprocal, exactly: 2^-k
+ * has biased exponent 2 * bias - (bias + k), so the reciprocal's bits are a
+ * constant minus x's bits. */
+__device__ __forceinline__ double recip_pow2(double x) {
+  return __builtin_bit_cast(double, 0x7fe0000000000000ull -
+                                        __builtin_bit_cast(unsigned long long, x));
+}
+__device__ __forceinline__ float recip_pow2(float x) {
+  return __builtin_bit_cast(float, 0x7f000000u - __builtin_bit_cast(unsigned int, x));
+}
+
+/* Reciprocal cell heights at a wave-uniform row. On power-of-two meshes
+ * (P2: every spacing a power of two, exponents well inside the normal range)
+ * from the scalar-loaded height by exponent arithmetic, bit for bit the IEEE
+ * quotient 1/cdy; otherwise the quotient (only P2 code reads these). */
+template <typename T, bool P2>
+struct RowRecip {
+  RowScalar<T> d;
+  /* (the flag matches RowGeo's constructor; these are always reciprocals) */
+  __device__ RowRecip(const T* arr, int base_row, int last_row, int lane, bool = true)
+      : d(arr, base_row, last_row, lane) {}
+  __device__ __forceinline__ T operator()(int r) const {
+    if constexpr (P2)
+      return recip_pow2(d(r));
+    else
+      return T(1.0) / d(r);
+  }
+};
+
 /* ---- boundary-strip writers (ghost writes only in edge items) ---- */
 template <typename T>
 __device__ __forceinline__ void mput_cell(bool edge, T* a, int nx, int ny, int i,
@@ -369,7 +397,7 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* rows of the x-updated fields for this lane's column: w[0] oldest */
   T wr[4], we[4];
@@ -526,7 +554,7 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q, rcdx_m = T(1.0) / cdx_m;
-  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2 of column qm */
   T wr[4], we[4], wv[3];
@@ -1182,7 +1210,10 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T iedx_q = g.iedx[qg], rcdx_q = T(1.0) / cdx_q;
   const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
-  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+  /* lane-held reciprocals in FP32: 2% faster there than RowRecip */
+  using Recip = typename std::conditional<std::is_same<T, float>::value, RowGeo<T>,
+                                          RowRecip<T, P2>>::type;
+  const Recip rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* y face k of the lane's column after pressure acceleration, from cell
    * rows k (rk, ek) and k-1 (rs, es) and mom_y at face k */
@@ -1442,7 +1473,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* x stage: u1 and m1 of cell row i from that row's u0, F_x, mx1, rho */
   auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
@@ -1636,7 +1667,7 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowGeo<T> rcdy(g.cdy, y0 - 4, ny, lane, true);
+  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* x stage: x flux in y at corner row gf from u0 rows gf-2 .. gf+1 */
   auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {

@@ -16,12 +16,12 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems);
+    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_mass_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems);
+    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_mass_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   }
@@ -35,7 +35,7 @@ hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double
   const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
   if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
-  const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems, kLagMaxWaves<T>);
+  const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
   hipLaunchKernelGGL((k_march_lagrangian<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                      (const double*)r.dtp, r, max_dt, first_step, ml);
   return hipGetLastError();
@@ -84,12 +84,12 @@ hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* 
   if (ml.nitems < 1 || !queue) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems, kMomMaxWaves<T>);
+    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_momxy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        dtp, queue, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems, kMomMaxWaves<T>);
+    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_momxy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        dtp, queue, ml);
   }
