@@ -493,7 +493,7 @@ int flowhip_ensemble_run(FlowHipEnsemble* E, int niters, int tt0, int pre_timest
   return niters;
 }
 
-/* Try every chunk height in {16, 24, 32, 48, 64}, work order (instance,
+/* Try every chunk height in {16, 20, 24, 32, 48, 64}, work order (instance,
  * largest first, round robin) and momentum schedule (fused, x then y) for
  * trial_steps steps from the current state; keep the fastest and put the
  * state back. */
@@ -506,7 +506,7 @@ int flowhip_ensemble_autotune(FlowHipEnsemble* E, int trial_steps, int tt0,
   HIPCHECK(hipMemcpy(E->snap, E->pool, E->pool_bytes, hipMemcpyDeviceToDevice));
   HIPCHECK(hipMemcpy((char*)E->snap + E->pool_bytes, E->d_dtp,
                      sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToDevice));
-  const int rows[5] = {16, 24, 32, 48, 64};
+  const int rows[] = {16, 20, 24, 32, 48, 64};
   double best = 1e300;
   int brow = E->chunk_rows, bord = E->order, bfm = E->fused_momentum;
   hipEvent_t a, b;
