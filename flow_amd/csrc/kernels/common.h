@@ -39,6 +39,17 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
+/* 1/x for a positive power of two x with a normal reciprocal, exactly: 2^-k
+ * has biased exponent 2 * bias - (bias + k), so the reciprocal's bits are a
+ * constant minus x's bits. */
+__host__ __device__ __forceinline__ double recip_pow2(double x) {
+  return __builtin_bit_cast(double, 0x7fe0000000000000ull -
+                                        __builtin_bit_cast(unsigned long long, x));
+}
+__host__ __device__ __forceinline__ float recip_pow2(float x) {
+  return __builtin_bit_cast(float, 0x7f000000u - __builtin_bit_cast(unsigned int, x));
+}
+
 /* Error-free transformation a + b = s + err (Knuth TwoSum). */
 __host__ __device__ __forceinline__ void two_sum(double a, double b, double& s,
                                                  double& err) {

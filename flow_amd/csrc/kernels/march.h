@@ -289,17 +289,6 @@ struct RowScalar {
   __device__ __forceinline__ T operator()(int r) const { return p[max(0, min(r, last))]; }
 };
 
-/* 1/x for a positive power of two x with a normal reciprocal, exactly: 2^-k
- * has biased exponent 2 * bias - (bias + k), so the reciprocal's bits are a
- * constant minus x's bits. */
-__device__ __forceinline__ double recip_pow2(double x) {
-  return __builtin_bit_cast(double, 0x7fe0000000000000ull -
-                                        __builtin_bit_cast(unsigned long long, x));
-}
-__device__ __forceinline__ float recip_pow2(float x) {
-  return __builtin_bit_cast(float, 0x7f000000u - __builtin_bit_cast(unsigned int, x));
-}
-
 /* Reciprocal cell heights at a wave-uniform row. On power-of-two meshes
  * (P2: every spacing a power of two, exponents well inside the normal range)
  * from the scalar-loaded height by exponent arithmetic, bit for bit the IEEE
