@@ -111,7 +111,7 @@ void flowhip_invalidate_sums(Mesh* mesh);
  * height gives bitwise the same result. */
 enum { FLOWHIP_K_LAGRANGIAN = 0, FLOWHIP_K_MASS = 1, FLOWHIP_K_MOMENTUM = 2 };
 static inline int flowhip_rows(const FlowHipCtx* h, int kernel, int fp32) {
-  static const int kRows[2][3] = {{28, 20, 16}, {28, 16, 20}};
+  static const int kRows[2][3] = {{24, 20, 16}, {28, 16, 20}};
   return h->march_rows > 0 ? h->march_rows : kRows[fp32 ? 1 : 0][kernel];
 }
 
