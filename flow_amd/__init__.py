@@ -7,7 +7,8 @@ computes the timestep itself, and importing the runtime fails loudly when the
 library has not been built.
 """
 from .native import (  # noqa: F401
-    ARRAY_NAMES, FUSED, MARCH, STAGEWISE, Mesh, MeshStruct, DeviceState,
+    ARRAY_NAMES, CFL_DEFINED, CFL_OMP3_SERIAL, FUSED, MARCH, STAGEWISE, Mesh,
+    MeshStruct, DeviceState,
     DeviceState32, Ensemble, lib,
     library_path,
 )

@@ -189,6 +189,7 @@ int main(int argc, char** argv) {
   /* main.c:68-115 */
   double wallclock = 0.0;
   int tt = step0;
+  int undefined_steps = 0, undefined_max = 0; /* steps with undefined CFL candidates */
   for (int k = 0; k < mesh.niters; ++k, ++tt) {
     printf("\nIteration %d\n", tt + 1);
     const double w0 = now_seconds();
@@ -198,6 +199,11 @@ int main(int argc, char** argv) {
     print_conservation(mesh.local_nx, mesh.local_ny, sd.density, sd.energy,
                        sd.reduce_array0, &mesh);
     wallclock += now_seconds() - w0;
+    const int und = flowhip_undefined_candidates(&mesh);
+    if (und) {
+      ++undefined_steps;
+      if (und > undefined_max) undefined_max = und;
+    }
     sim_time += mesh.dt;
     if (sim_time >= mesh.sim_end) {
       printf("reached end of simulation time\n");
@@ -213,6 +219,14 @@ int main(int argc, char** argv) {
                                "density", tt, sim_time);
   }
   barrier();
+  /* on stderr, so that stdout keeps main.c's lines */
+  if (undefined_steps)
+    fprintf(stderr,
+            "Warning: %d step(s) met cells with negative energy (undefined CFL "
+            "candidates, at most %d in one step); their dt followed the %s rule.\n",
+            undefined_steps, undefined_max,
+            flowhip_get_cfl_rule(&mesh) == FLOWHIP_CFL_OMP3_SERIAL ? "omp3-serial"
+                                                                   : "defined-minimum");
 
   const int result = validate_run(&mesh, opt.params, sd.density, sd.energy);
   printf("Wallclock %.4fs, Elapsed Simulation Time %.4fs\n", wallclock, sim_time);

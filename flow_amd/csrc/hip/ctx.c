@@ -22,6 +22,15 @@ static void schedule_from_env(FlowHipCtx* h) {
     else
       TERMINATE("FLOWHIP_SCHEDULE must be stagewise, fused or march, not '%s'", s);
   }
+  const char* c = getenv("FLOWHIP_CFL_RULE");
+  if (c && *c) {
+    if (!strcmp(c, "defined"))
+      h->cfl_rule = FLOWHIP_CFL_DEFINED;
+    else if (!strcmp(c, "omp3-serial"))
+      h->cfl_rule = FLOWHIP_CFL_OMP3_SERIAL;
+    else
+      TERMINATE("FLOWHIP_CFL_RULE must be defined or omp3-serial, not '%s'", c);
+  }
   const char* r = getenv("FLOWHIP_MARCH_ROWS");
   if (r && *r) {
     char* end = NULL;
@@ -89,6 +98,7 @@ void flowhip_ctx_destroy(Mesh* mesh) {
   hipFree(h->inv_edgedx);
   hipFree(h->inv_edgedy);
   hipFree(h->geo32);
+  hipFree(h->cfl_buf);
   hipFree(h->red.partials);
   hipFree(h->red.ticket);
   hipFree(h->red.dtp);
@@ -173,6 +183,33 @@ FkMesh flowhip_fkmesh(const Mesh* mesh) {
 void flowhip_invalidate_sums(Mesh* mesh) {
   if (mesh && mesh->hip) ((FlowHipCtx*)mesh->hip)->sums_valid = 0;
 }
+
+FkReduce flowhip_lagrangian_red(FlowHipCtx* h, int nx, int ny) {
+  FkReduce r = h->red;
+  r.cfl = NULL;
+  if (h->cfl_rule != FLOWHIP_CFL_OMP3_SERIAL) return r;
+  const size_t need = (size_t)nx * (size_t)ny;
+  if (h->cfl_len < need) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    hipFree(h->cfl_buf);
+    h->cfl_buf = NULL;
+    h->cfl_len = 0;
+    HIPCHECK(hipMalloc((void**)&h->cfl_buf, sizeof(double) * need));
+    h->cfl_len = need;
+  }
+  r.cfl = h->cfl_buf;
+  return r;
+}
+
+int flowhip_set_cfl_rule(Mesh* mesh, int rule) {
+  if (rule != FLOWHIP_CFL_DEFINED && rule != FLOWHIP_CFL_OMP3_SERIAL) return -1;
+  flowhip_ctx(mesh)->cfl_rule = rule;
+  return 0;
+}
+
+int flowhip_get_cfl_rule(Mesh* mesh) { return flowhip_ctx(mesh)->cfl_rule; }
+
+int flowhip_undefined_candidates(Mesh* mesh) { return flowhip_ctx(mesh)->h_status[1]; }
 
 static int timer_index(FlowHipCtx* h, const char* name) {
   for (int k = 0; k < h->ntimers; ++k)

@@ -441,11 +441,11 @@ static double now_seconds(void) {
 
 /* main.c:64-115 for every instance at once, without sim_end (every
  * instance takes niters steps). dt_pairs: [instance][niters+1][2] (the
- * pre-loop pair, then after each step); mass, energy: [instance][niters];
- * any may be NULL. Returns niters. */
+ * pre-loop pair, then after each step); mass, energy, undefined:
+ * [instance][niters]; any may be NULL. Returns niters. */
 int flowhip_ensemble_run(FlowHipEnsemble* E, int niters, int tt0, int pre_timestep,
                          double* dt_pairs, double* mass, double* energy,
-                         double* wall_seconds) {
+                         double* wall_seconds, int* undefined) {
   if (!E || niters < 0) return -1;
   const int n = E->ninst;
   ens_reserve_history(E, niters > 0 ? niters : 1);
@@ -489,6 +489,15 @@ int flowhip_ensemble_run(FlowHipEnsemble* E, int niters, int tt0, int pre_timest
     }
     free(hd);
     free(hs);
+  }
+  if (niters > 0 && undefined) {
+    const size_t m = (size_t)niters * (size_t)n;
+    int* hn = (int*)malloc(sizeof(int) * m);
+    if (!hn) TERMINATE("ensemble: out of host memory");
+    HIPCHECK(hipMemcpy(hn, E->hist.nan, sizeof(int) * m, hipMemcpyDeviceToHost));
+    for (int k = 0; k < niters; ++k)
+      for (int i = 0; i < n; ++i) undefined[(size_t)i * niters + k] = hn[(size_t)k * n + i];
+    free(hn);
   }
   return niters;
 }

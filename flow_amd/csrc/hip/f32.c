@@ -73,14 +73,17 @@ static void pull32(FlowHipCtx* h, Mesh* mesh, int with_sums) {
   if (with_sums)
     HIPCHECK(hipMemcpyAsync(h->h_stage + 2, h->red.sums, 2 * sizeof(double),
                             hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(h->h_status, h->red.status, 2 * sizeof(int),
+                          hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   flowhip_timer_collect(h);
   mesh->dt = h->h_stage[0];
   mesh->dt_h = h->h_stage[1];
   if (!isfinite(mesh->dt) || !(mesh->dt > 0.0) || !isfinite(mesh->dt_h))
     TERMINATE("hip kernel set (FP32 state): non-finite or non-positive "
-              "timestep (dt=%.12e dt_h=%.12e): the state has diverged",
-              mesh->dt, mesh->dt_h);
+              "timestep (dt=%.12e dt_h=%.12e, %d undefined CFL candidates): "
+              "the state has diverged",
+              mesh->dt, mesh->dt_h, h->h_status[1]);
 }
 
 /* One step of omp3/flow.c:10-52 on FP32 state, in the march schedule's four
@@ -114,7 +117,7 @@ static double now_seconds(void) {
 /* main.c:64-115 on FP32 state; the arguments mean what flowhip_run's do. */
 int flowhip_run_f32(Mesh* mesh, float** a, int niters, int tt0,
                     int pre_timestep, double* dt_pairs, double* mass,
-                    double* energy, double* wall_seconds) {
+                    double* energy, double* wall_seconds, int* undefined) {
   if (niters < 0 || !a) return -1;
   if (mesh->local_nx - 2 * mesh->pad < 2 || mesh->local_ny - 2 * mesh->pad < 2)
     TERMINATE("flowhip_run_f32: the mesh needs at least 2x2 interior cells");
@@ -132,6 +135,7 @@ int flowhip_run_f32(Mesh* mesh, float** a, int niters, int tt0,
     dt_pairs[0] = mesh->dt;
     dt_pairs[1] = mesh->dt_h;
   }
+  if (undefined) undefined[0] = pre_timestep ? h->h_status[1] : 0;
   double wall = 0.0, elapsed = 0.0;
   int k;
   for (k = 0; k < niters; ++k) {
@@ -152,6 +156,7 @@ int flowhip_run_f32(Mesh* mesh, float** a, int niters, int tt0,
     }
     if (mass) mass[k] = h->h_stage[2];
     if (energy) energy[k] = h->h_stage[3];
+    if (undefined) undefined[k + 1] = h->h_status[1];
     elapsed += mesh->dt;
     if (elapsed >= mesh->sim_end) {
       ++k;

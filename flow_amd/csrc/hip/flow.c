@@ -95,9 +95,14 @@ void set_timestep(const int nx, const int ny, double* Qxx, double* Qyy,
   m.celldx = celldx;
   m.celldy = celldy;
   push_dt(h, mesh);
-  FK_LAUNCH(h, "timestep",
-            fk_timestep(m, Qxx, Qyy, rho, e, h->red, mesh->max_dt, first_step,
-                        h->stream));
+  if (h->cfl_rule == FLOWHIP_CFL_OMP3_SERIAL)
+    FK_LAUNCH(h, "timestep_serial",
+              fk_timestep_serial(m, NULL, Qxx, Qyy, rho, e, h->red, mesh->max_dt,
+                                 first_step, h->stream));
+  else
+    FK_LAUNCH(h, "timestep",
+              fk_timestep(m, Qxx, Qyy, rho, e, h->red, mesh->max_dt, first_step,
+                          h->stream));
   pull_dt(h, mesh);
 }
 

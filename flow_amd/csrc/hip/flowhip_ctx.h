@@ -58,13 +58,19 @@ typedef struct FlowHipCtx {
   int* h_status;        /* pinned */
   int schedule;
   int march_rows;       /* chunk height of every march kernel; 0: each its own */
+  /* rule for undefined CFL candidates (FLOWHIP_CFL_*); under the serial
+   * rule the fused Lagrangian kernels store every candidate in cfl_buf
+   * (cfl_len doubles, cell layout) for fk_timestep_serial */
+  int cfl_rule;
+  double* cfl_buf;
+  size_t cfl_len;
   int use_graphs;
   /* captured steps: [0] even tt, [1] odd tt, [2] tt == 0; each valid for
    * the launch arguments, schedule and chunk height it was captured with */
   hipGraphExec_t graph[3];
   int graph_ok[3];
   FlowHipStep graph_step[3];
-  int graph_sched[3], graph_rows[3];
+  int graph_sched[3], graph_rows[3], graph_rule[3];
   int graph_warm;
   /* conservation sums fused into the last advection kernel, valid while
    * the arrays they were taken from are unchanged */
@@ -102,6 +108,11 @@ void flowhip_timer_collect(FlowHipCtx* h);
 
 /* Invalidate cached conservation sums (any write to device state). */
 void flowhip_invalidate_sums(Mesh* mesh);
+
+/* The reduction scratch for an FP64 Lagrangian launch on an nx x ny
+ * (local) mesh: h->red, with .cfl pointing at the candidate buffer (grown
+ * to nx * ny doubles) under the serial CFL rule. */
+FkReduce flowhip_lagrangian_red(FlowHipCtx* h, int nx, int ny);
 
 /* Chunk height of one march kernel. An explicit height (flowhip_set_march_rows,
  * FLOWHIP_MARCH_ROWS) applies to all of them; by default each kernel uses

@@ -21,16 +21,23 @@ void flowhip_step_fused(FlowHipCtx* h, const FlowHipStep* st) {
   double* my1 = st->Qyy;
   double* u0 = st->uF_x;
   double* v0 = st->uF_y;
+  /* under the serial CFL rule the Lagrangian kernel stores its candidates
+   * and fk_timestep_serial folds them in omp3's order */
+  const FkReduce lr = flowhip_lagrangian_red(h, m.nx, m.ny);
   if (h->schedule == FLOWHIP_MARCH)
     FK_LAUNCH(h, "lagrangian",
               fk_march_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1,
-                                  my1, u0, v0, e1, rho0, h->red, st->max_dt,
+                                  my1, u0, v0, e1, rho0, lr, st->max_dt,
                                   st->tt == 0, flowhip_rows(h, FLOWHIP_K_LAGRANGIAN, 0),
                                   s));
   else
     FK_LAUNCH(h, "lagrangian",
               fk_lagrangian(m, st->rho, st->e, st->mom_x, st->mom_y, mx1, my1,
-                            u0, v0, e1, rho0, h->red, st->max_dt, st->tt == 0, s));
+                            u0, v0, e1, rho0, lr, st->max_dt, st->tt == 0, s));
+  if (lr.cfl)
+    FK_LAUNCH(h, "timestep_serial",
+              fk_timestep_serial(m, lr.cfl, NULL, NULL, NULL, NULL, lr, st->max_dt,
+                                 st->tt == 0, s));
   if (h->schedule == FLOWHIP_MARCH)
     FK_LAUNCH(h, "advect_mass_energy",
               fk_march_mass_energy(m, st->tt, u0, v0, rho0, e1, st->rho, st->e,

@@ -40,7 +40,7 @@ static double now_seconds(void) {
 /* main.c:64-115 */
 int flowhip_run(Mesh* mesh, double** a, int niters, int tt0, int pre_timestep,
                 double* dt_pairs, double* mass, double* energy,
-                double* wall_seconds) {
+                double* wall_seconds, int* undefined) {
   if (niters < 0 || !a) return -1;
   const int nx = mesh->local_nx, ny = mesh->local_ny;
   double *P = a[0], *rho = a[1], *rho_old = a[2], *e = a[3], *u = a[4],
@@ -54,6 +54,7 @@ int flowhip_run(Mesh* mesh, double** a, int niters, int tt0, int pre_timestep,
     dt_pairs[0] = mesh->dt;
     dt_pairs[1] = mesh->dt_h;
   }
+  if (undefined) undefined[0] = pre_timestep ? flowhip_undefined_candidates(mesh) : 0;
   double wall = 0.0, elapsed = 0.0;
   int k;
   for (k = 0; k < niters; ++k) {
@@ -69,6 +70,7 @@ int flowhip_run(Mesh* mesh, double** a, int niters, int tt0, int pre_timestep,
       dt_pairs[2 * (k + 1)] = mesh->dt;
       dt_pairs[2 * (k + 1) + 1] = mesh->dt_h;
     }
+    if (undefined) undefined[k + 1] = flowhip_undefined_candidates(mesh);
     double ms, es;
     flowhip_last_sums(mesh, &ms, &es);
     if (mass) mass[k] = ms;

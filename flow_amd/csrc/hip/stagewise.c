@@ -61,9 +61,14 @@ void flowhip_step_stagewise(FlowHipCtx* h, const FlowHipStep* st) {
   BND(h, nx, ny, st->e, NO_INVERT);
 
   /* the only global dependency: the new (dt, dt_h), left on the device */
-  FK_LAUNCH(h, "timestep",
-            fk_timestep(m, st->Qxx, st->Qyy, st->rho, st->e, h->red,
-                        st->max_dt, st->tt == 0, s));
+  if (h->cfl_rule == FLOWHIP_CFL_OMP3_SERIAL)
+    FK_LAUNCH(h, "timestep_serial",
+              fk_timestep_serial(m, NULL, st->Qxx, st->Qyy, st->rho, st->e, h->red,
+                                 st->max_dt, st->tt == 0, s));
+  else
+    FK_LAUNCH(h, "timestep",
+              fk_timestep(m, st->Qxx, st->Qyy, st->rho, st->e, h->red,
+                          st->max_dt, st->tt == 0, s));
 
   /* advection of mass and energy (omp3/flow.c:293-323) */
   FK_LAUNCH(h, "copy_rho_old",
@@ -162,7 +167,8 @@ void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {
   k.tt = slot == 2 ? 0 : (slot == 1 ? 1 : 2);
   if (!(h->graph_ok[slot] && same_step(&h->graph_step[slot], &k) &&
         h->graph_sched[slot] == h->schedule &&
-        h->graph_rows[slot] == h->march_rows)) {
+        h->graph_rows[slot] == h->march_rows &&
+        h->graph_rule[slot] == h->cfl_rule)) {
     if (!h->graph_warm) {
       /* the first step runs directly: launch shapes are sized then */
       enqueue_direct(h, st);
@@ -171,6 +177,8 @@ void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {
     }
     if (h->graph_ok[slot]) HIPCHECK(hipGraphExecDestroy(h->graph[slot]));
     h->graph_ok[slot] = 0;
+    /* no allocation may happen inside the capture */
+    (void)flowhip_lagrangian_red(h, st->m.nx, st->m.ny);
     hipGraph_t g;
     HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_direct(h, &k);
@@ -181,6 +189,7 @@ void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {
     h->graph_step[slot] = k;
     h->graph_sched[slot] = h->schedule;
     h->graph_rows[slot] = h->march_rows;
+    h->graph_rule[slot] = h->cfl_rule;
   }
   HIPCHECK(hipGraphLaunch(h->graph[slot], h->stream));
   /* host-side state of a direct launch: the graph's advection kernel left

@@ -120,6 +120,7 @@ struct LagIO {
   T* __restrict__ v0;      /* y velocity after the viscous step, + ghosts */
   T* __restrict__ e1;      /* energy after shock heating, + ghosts */
   T* __restrict__ rho_old; /* copy of rho over every cell (omp3/flow.c:302-305) */
+  double* __restrict__ cfl; /* NULL, or every interior CFL candidate */
 };
 
 template <typename T>
@@ -264,6 +265,7 @@ __global__ __launch_bounds__(kNT, FLOWHIP_MIN_WAVES) void k_lagrangian(Geo<T> g,
           local_min = lesser(local_min, cand);
         else
           ++nan_count;
+        if (io.cfl) io.cfl[c] = cand;
       });
       __syncthreads();
     };
@@ -932,7 +934,7 @@ hipError_t fk_lagrangian(FkMesh m, const double* rho, const double* e,
                          double* my1, double* u0, double* v0, double* e1,
                          double* rho_old, FkReduce r, double max_dt,
                          int first_step, hipStream_t s) {
-  LagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
+  LagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old, r.cfl};
   return launch_lagrangian<double>(geo64(m), io, r, max_dt, first_step, s);
 }
 

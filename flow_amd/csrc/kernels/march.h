@@ -1158,6 +1158,7 @@ struct MLagIO {
   T* __restrict__ v0;      /* y velocity after the viscous step, + ghosts */
   T* __restrict__ e1;      /* energy after shock heating, + ghosts */
   T* __restrict__ rho_old; /* copy of rho over every cell */
+  double* __restrict__ cfl; /* NULL, or every interior CFL candidate (FP64) */
 };
 
 /* EOS, pressure acceleration, artificial viscosity, shock heating, the CFL
@@ -1365,6 +1366,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
         local_min = lesser(local_min, cand);
       else
         ++nan_count;
+      if (io.cfl) io.cfl[c] = cand;
     }
 
     /* advance the windows */

@@ -148,7 +148,7 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
                                double* my1, double* u0, double* v0, double* e1,
                                double* rho_old, FkReduce r, double max_dt,
                                int first_step, int chunk_rows, hipStream_t s) {
-  MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
+  MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old, r.cfl};
   return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
                                          chunk_rows, m.pow2, s);
 }

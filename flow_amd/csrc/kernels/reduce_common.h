@@ -65,7 +65,10 @@ __device__ __forceinline__ CSum block_csum(CSum v) {
 }
 
 /* Per-block CFL minimum -> last block -> (dt, dt_h) update of
- * omp3/flow.c:103-105 and the undefined-candidate count. */
+ * omp3/flow.c:103-105 and the undefined-candidate count. With r.cfl set the
+ * candidates were also stored for k_timestep_serial, which then does the
+ * update and publishes the count; this only leaves the minimum over the
+ * defined candidates in r.sums[2]. */
 /* kSquared: the candidates are squared time steps (the FP32-state form);
  * the minimum's square root is taken once, here. */
 template <int NT, bool kSquared = false>
@@ -81,13 +84,15 @@ __device__ void finish_timestep(double local_min, int nan_count, FkReduce r,
   g = block_min<NT>(g);
   if (kSquared) g = __builtin_sqrt(g);
   if (threadIdx.x == 0) {
-    const double dt_h = r.dtp[1];
-    const double final_min = lesser(g, kCM * dt_h);
-    const double dt = 0.5 * (kCT * final_min + dt_h);
-    r.dtp[0] = dt;
-    r.dtp[1] = first_step ? dt : kCT * final_min;
+    if (!r.cfl) {
+      const double dt_h = r.dtp[1];
+      const double final_min = lesser(g, kCM * dt_h);
+      const double dt = 0.5 * (kCT * final_min + dt_h);
+      r.dtp[0] = dt;
+      r.dtp[1] = first_step ? dt : kCT * final_min;
+      r.status[1] = atomicExch(r.status, 0);
+    }
     r.sums[2] = g;
-    r.status[1] = atomicExch(r.status, 0);
     *r.ticket = 0u;
   }
 }

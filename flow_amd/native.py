@@ -10,6 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.environ.get("FLOWHIP_LIBRARY") or os.path.join(_HERE, "lib", "libflowhip.so")
 
 STAGEWISE, FUSED, MARCH = 0, 1, 2
+# Rules for undefined CFL candidates (flow_hip.h FLOWHIP_CFL_*).
+CFL_DEFINED, CFL_OMP3_SERIAL = 0, 1
 NO_INVERT, INVERT_X, INVERT_Y = 0, 1, 2
 RECV, SEND = 0, 1
 
@@ -67,7 +69,10 @@ def lib():
         "flowhip_timing_entry": (i, [mp, i, ctypes.c_char_p, i, _dp,
                                      ctypes.POINTER(ctypes.c_long)]),
         "flowhip_timing_reset": (i, [mp]),
-        "flowhip_run": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp]),
+        "flowhip_run": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp, vp]),
+        "flowhip_set_cfl_rule": (i, [mp, i]),
+        "flowhip_get_cfl_rule": (i, [mp]),
+        "flowhip_undefined_candidates": (i, [mp]),
         "flowhip_device_sync": (i, []),
         "flowhip_memset_zero": (i, [vp, sz]),
         "flowhip_copy": (i, [vp, vp, sz, i]),
@@ -79,7 +84,7 @@ def lib():
         "flowhip_f32_narrow": (i, [vp, vp, sz]),
         "flowhip_f32_widen": (i, [vp, vp, sz]),
         "flowhip_f32_boundary": (i, [mp, vp, i, i, i]),
-        "flowhip_run_f32": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp]),
+        "flowhip_run_f32": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp, vp]),
         "flowhip_ensemble_new": (vp, [i, ctypes.POINTER(i), ctypes.POINTER(i), _dp, _dp, _dp, _dp, i]),
         "flowhip_ensemble_delete": (None, [vp]),
         "flowhip_ensemble_size": (i, [vp]),
@@ -92,7 +97,7 @@ def lib():
         "flowhip_ensemble_set_schedule": (i, [vp, i, i, i]),
         "flowhip_ensemble_get_schedule": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
         "flowhip_ensemble_autotune": (i, [vp, i, i, _dp]),
-        "flowhip_ensemble_run": (i, [vp, i, i, i, vp, vp, vp, _dp]),
+        "flowhip_ensemble_run": (i, [vp, i, i, i, vp, vp, vp, _dp, vp]),
         "allocate_data": (sz, [ctypes.POINTER(vp), sz]),
         "deallocate_data": (None, [vp]),
         "handle_boundary_2d": (None, [i, i, mp, vp, i, i]),
@@ -164,6 +169,18 @@ class Mesh:
 
     def set_graphs(self, on):
         lib().flowhip_set_graphs(self.ptr, 1 if on else 0)
+
+    def set_cfl_rule(self, rule):
+        """CFL_DEFINED or CFL_OMP3_SERIAL (flow_hip.h)."""
+        if lib().flowhip_set_cfl_rule(self.ptr, rule) != 0:
+            raise ValueError(f"bad CFL rule {rule}")
+
+    def cfl_rule(self):
+        return lib().flowhip_get_cfl_rule(self.ptr)
+
+    def undefined_candidates(self):
+        """Undefined CFL candidates met by the last dt computation."""
+        return lib().flowhip_undefined_candidates(self.ptr)
 
     def set_timing(self, on):
         lib().flowhip_set_timing(self.ptr, 1 if on else 0)
@@ -253,14 +270,18 @@ class DeviceState:
 
     def run(self, niters, tt0=0, pre_timestep=True):
         """main.c:64-115 through flowhip_run. Returns (steps, dt_pairs,
-        mass, energy, wall_seconds)."""
+        mass, energy, wall_seconds); self.undefined then holds the undefined
+        CFL candidates of the pre-loop call and of each step."""
         dt = np.zeros(2 * (niters + 1))
         mass = np.zeros(max(niters, 1))
         energy = np.zeros(max(niters, 1))
+        und = np.zeros(niters + 1, dtype=np.int32)
         wall = ctypes.c_double()
         steps = lib().flowhip_run(self.mesh.ptr, self.array_ptrs(), niters, tt0,
                                   1 if pre_timestep else 0, _ptr(dt),
-                                  _ptr(mass), _ptr(energy), ctypes.byref(wall))
+                                  _ptr(mass), _ptr(energy), ctypes.byref(wall),
+                                  und.ctypes.data_as(ctypes.c_void_p))
+        self.undefined = und[: steps + 1].copy()
         return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
                 energy[:steps], wall.value)
 
@@ -325,10 +346,13 @@ class DeviceState32:
         dt = np.zeros(2 * (niters + 1))
         mass = np.zeros(max(niters, 1))
         energy = np.zeros(max(niters, 1))
+        und = np.zeros(niters + 1, dtype=np.int32)
         wall = ctypes.c_double()
         steps = lib().flowhip_run_f32(self.mesh.ptr, self.array_ptrs(), niters,
                                       tt0, 1 if pre_timestep else 0, _ptr(dt),
-                                      _ptr(mass), _ptr(energy), ctypes.byref(wall))
+                                      _ptr(mass), _ptr(energy), ctypes.byref(wall),
+                                      und.ctypes.data_as(ctypes.c_void_p))
+        self.undefined = und[: steps + 1].copy()
         return (steps, dt[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
                 energy[:steps], wall.value)
 
@@ -408,9 +432,12 @@ class Ensemble:
         dt = np.zeros(self.n * 2 * (niters + 1))
         mass = np.zeros(max(self.n * niters, 1))
         energy = np.zeros(max(self.n * niters, 1))
+        und = np.zeros(max(self.n * niters, 1), dtype=np.int32)
         wall = ctypes.c_double()
         lib().flowhip_ensemble_run(self.ptr, niters, tt0, 1 if pre_timestep else 0,
-                                   _ptr(dt), _ptr(mass), _ptr(energy), ctypes.byref(wall))
+                                   _ptr(dt), _ptr(mass), _ptr(energy), ctypes.byref(wall),
+                                   und.ctypes.data_as(ctypes.c_void_p))
+        self.undefined = und[: self.n * niters].reshape(self.n, niters).copy()
         return (dt.reshape(self.n, niters + 1, 2),
                 mass[: self.n * niters].reshape(self.n, niters),
                 energy[: self.n * niters].reshape(self.n, niters), wall.value)

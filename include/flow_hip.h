@@ -44,6 +44,29 @@ int flowhip_set_march_rows(Mesh* mesh, int rows);
  * (flowhip_set_timing) launches directly. */
 int flowhip_set_graphs(Mesh* mesh, int on);
 
+/* Rules for set_timestep's undefined CFL candidates (cells with negative
+ * energy or zero density, whose candidate omp3/flow.c:87-94 makes NaN).
+ * omp3 folds the candidates with min(a, b) = a < b ? a : b under an OpenMP
+ * min reduction: a NaN makes the running value NaN and the next candidate
+ * replaces it, so each thread forgets its candidates before its last NaN,
+ * and the result depends on OMP_NUM_THREADS.
+ *   FLOWHIP_CFL_DEFINED (default): the minimum over the defined candidates
+ *     and max_dt; order- and thread-independent, and equal to omp3's
+ *     result whenever no candidate is undefined.
+ *   FLOWHIP_CFL_OMP3_SERIAL: omp3 on one thread, bit for bit: the fold in
+ *     row-major order from +inf, merged into max_dt by OpenMP's combiner
+ *     out = in < out ? in : out. Costs a candidate store and one more
+ *     launch per step (FP64 single-mesh forms only).
+ * The environment may set it for callers that only see flow_interface.h:
+ * FLOWHIP_CFL_RULE=defined|omp3-serial. */
+#define FLOWHIP_CFL_DEFINED 0
+#define FLOWHIP_CFL_OMP3_SERIAL 1
+int flowhip_set_cfl_rule(Mesh* mesh, int rule);
+int flowhip_get_cfl_rule(Mesh* mesh);
+/* Undefined CFL candidates met by the last set_timestep or solve_hydro_2d
+ * call (each is left out of, or restarts, the minimum as the rule says). */
+int flowhip_undefined_candidates(Mesh* mesh);
+
 /* Per-kernel timing with hipEvents on the kernel set's stream. */
 int flowhip_set_timing(Mesh* mesh, int on);
 int flowhip_timing_count(Mesh* mesh);
@@ -58,11 +81,13 @@ int flowhip_timing_reset(Mesh* mesh);
  * uF_x uF_y vF_x vF_y (device). dt_pairs receives 2*(niters+1) values
  * ((dt, dt_h) after the pre-loop call, then after each step); mass/energy
  * receive the per-step totals; wall_seconds the time spent in the steps.
- * Any output pointer may be NULL. Returns the number of steps taken.
- * pre_timestep=0 skips the pre-loop call (to continue a run). */
+ * undefined receives niters+1 counts of undefined CFL candidates (the
+ * pre-loop call's, then each step's; 0 when pre_timestep is 0). Any output
+ * pointer may be NULL. Returns the number of steps taken. pre_timestep=0
+ * skips the pre-loop call (to continue a run). */
 int flowhip_run(Mesh* mesh, double** arrays, int niters, int tt0,
                 int pre_timestep, double* dt_pairs, double* mass,
-                double* energy, double* wall_seconds);
+                double* energy, double* wall_seconds, int* undefined);
 
 /* ---- FP32-state precision form ----
  * The 16 arrays of solve_hydro_2d (same roles, layouts and (L+1)^2 sizes)
@@ -84,7 +109,7 @@ int flowhip_f32_boundary(Mesh* mesh, float* arr, int nxp, int nyp, int invert);
  * totals are FP64. */
 int flowhip_run_f32(Mesh* mesh, float** arrays, int niters, int tt0,
                     int pre_timestep, double* dt_pairs, double* mass,
-                    double* energy, double* wall_seconds);
+                    double* energy, double* wall_seconds, int* undefined);
 
 /* ---- ensemble (batching) form ----
  * ninst independent meshes of uneven sizes (global_nx[i] x global_ny[i]
@@ -127,10 +152,11 @@ int flowhip_ensemble_get_schedule(FlowHipEnsemble* ens, int* chunk_rows, int* or
 int flowhip_ensemble_autotune(FlowHipEnsemble* ens, int trial_steps, int tt0,
                               double* best_ms_per_step);
 /* niters steps of every instance; dt_pairs [inst][niters+1][2], mass and
- * energy [inst][niters] (any may be NULL). Returns niters. */
+ * energy [inst][niters], undefined CFL candidates per step [inst][niters]
+ * (any may be NULL). Returns niters. */
 int flowhip_ensemble_run(FlowHipEnsemble* ens, int niters, int tt0,
                          int pre_timestep, double* dt_pairs, double* mass,
-                         double* energy, double* wall_seconds);
+                         double* energy, double* wall_seconds, int* undefined);
 
 /* Device helpers for callers that manage their own buffers. */
 int flowhip_device_sync(void);

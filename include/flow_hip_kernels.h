@@ -57,6 +57,11 @@ typedef struct FkReduce {
   /* FK_TOTALS_WORDS exact fixed-point accumulators of the mass/energy
    * march's totals; zero between launches (the last block clears them). */
   unsigned long long* totals;
+  /* NULL, or a cell-layout array (nx * ny doubles) that the FP64
+   * Lagrangian kernels fill with every interior CFL candidate. When set,
+   * those kernels leave the (dt, dt_h) update and the status count to a
+   * following fk_timestep_serial launch that reads the candidates. */
+  double* cfl;
 } FkReduce;
 
 #define FK_MARCH_QUEUE_WORDS 512
@@ -98,6 +103,18 @@ hipError_t fk_shock_heating(FkMesh m, const double* dtp, double* e,
 hipError_t fk_timestep(FkMesh m, const double* Qxx, const double* Qyy,
                        const double* rho, const double* e, FkReduce r,
                        double max_dt, int first_step, hipStream_t s);
+/* omp3/flow.c:77-105 with the candidates folded as one OpenMP thread folds
+ * them: every interior candidate in row-major order with min(a, b) = a < b ?
+ * a : b from +inf (a NaN candidate makes the running value NaN and the next
+ * candidate replaces it), then merged into max_dt by OpenMP's min combiner
+ * out = in < out ? in : out; then the (dt, dt_h) update. Candidates come from
+ * cfl (cell layout, as left by a Lagrangian kernel with FkReduce.cfl set),
+ * or, with cfl NULL, are computed from e, Qxx, Qyy and rho; undefined
+ * candidates are counted into r.status in the second case only. */
+hipError_t fk_timestep_serial(FkMesh m, const double* cfl, const double* Qxx,
+                              const double* Qyy, const double* rho,
+                              const double* e, FkReduce r, double max_dt,
+                              int first_step, hipStream_t s);
 /* omp3/flow.c:338-404 / 454-521 (dt = dtp[0], dt_h = dtp[1]) */
 hipError_t fk_x_flux(FkMesh m, const double* dtp, const double* rho,
                      const double* e, const double* u, double* F_x,

@@ -13,7 +13,8 @@
  * (main.c:90-91). The wall time covers solve + conservation of every step,
  * as main.c:80-93 times it. */
 int orc_run(OrcMesh* m, double** a, int niters, double* dt_seq,
-            double* mass_seq, double* energy_seq, double* wall_seconds) {
+            double* mass_seq, double* energy_seq, double* wall_seconds,
+            int* undefined_seq) {
   double *P = a[0], *rho = a[1], *rho_old = a[2], *e = a[3], *u = a[4],
          *v = a[5], *mom_x = a[6], *mom_y = a[7], *Qxx = a[8], *Qyy = a[9],
          *F_x = a[10], *F_y = a[11], *uF_x = a[12], *uF_y = a[13],
@@ -23,6 +24,7 @@ int orc_run(OrcMesh* m, double** a, int niters, double* dt_seq,
     dt_seq[0] = m->dt;
     dt_seq[1] = m->dt_h;
   }
+  if (undefined_seq) undefined_seq[0] = orc_last_undefined();
   double wall = 0.0, elapsed = 0.0;
   int tt;
   for (tt = 0; tt < niters; ++tt) {
@@ -42,6 +44,7 @@ int orc_run(OrcMesh* m, double** a, int niters, double* dt_seq,
       dt_seq[2 * (tt + 1) + 1] = m->dt_h;
     }
     if (mass_seq) mass_seq[tt] = mass;
+    if (undefined_seq) undefined_seq[tt + 1] = orc_last_undefined();
     if (energy_seq) energy_seq[tt] = energy;
     elapsed += m->dt;
     if (elapsed >= m->sim_end) {

@@ -133,32 +133,100 @@ void orc_shock_heating_and_work(int nx, int ny, const OrcMesh* m, double dt_h,
   orc_handle_boundary_2d(nx, ny, pad, e, ORC_NO_INVERT);
 }
 
-/* omp3/flow.c:73-106. One rank: reduce_all_min is the identity. */
+/* The CFL candidate of one cell, omp3/flow.c:87-94. A cell with negative
+ * energy (or zero density) gives NaN: sqrt of a negative, or 0/0. */
+static inline double cfl_candidate(const double* Qxx, const double* Qyy,
+                                   const double* rho, const double* e,
+                                   const OrcMesh* m, int nx, int i, int j) {
+  const int c = i * nx + j;
+  const double c_s = sqrt(GAM * (GAM - 1.0) * e[c]);
+  const double dtx = m->celldx[j] / sqrt(c_s * c_s + 2.0 * Qxx[c] / rho[c]);
+  const double dty = m->celldy[i] / sqrt(c_s * c_s + 2.0 * Qyy[c] / rho[c]);
+  return MIN(dtx, dty);
+}
+
+static int cfl_rule = ORC_CFL_DEFINED;
+static int last_undefined = 0;
+void orc_set_cfl_rule(int rule) { cfl_rule = rule; }
+int orc_get_cfl_rule(void) { return cfl_rule; }
+int orc_last_undefined(void) { return last_undefined; }
+
+/* omp3/flow.c:100-105: the clamp and the (dt, dt_h) recurrence. */
+static void timestep_update(OrcMesh* m, double global_min, int first_step) {
+  const double final_min = MIN(global_min, C_M * m->dt_h);
+  m->dt = 0.5 * (C_T * final_min + m->dt_h);
+  m->dt_h = first_step ? m->dt : C_T * final_min;
+}
+
+/* omp3/flow.c:73-106. One rank: reduce_all_min is the identity. Returns the
+ * minimum before the clamp.
+ *
+ * The two rules differ only where a candidate is NaN. omp3's loop folds the
+ * candidates with min(a, b) = a < b ? a : b. A NaN candidate makes the
+ * running value NaN (a < NaN is false), and the next candidate replaces that
+ * NaN (NaN < b is false), so every candidate before the last NaN is
+ * forgotten, and a NaN in the last cell leaves NaN. Under
+ * reduction(min : local_min_dt) each OpenMP thread folds its own rows from
+ * +inf, and the threads' values are merged into local_min_dt = max_dt by
+ * OpenMP's min combiner, out = in < out ? in : out, which passes over a
+ * thread's NaN (gcc's merge does so when built for AVX, as the reference's
+ * -march=native and this file's -march=x86-64-v3 build it; built for
+ * baseline SSE2 it propagates the NaN instead). So with NaN candidates the
+ * result depends on the thread count (orc_set_timestep_literal).
+ *
+ *   ORC_CFL_OMP3_SERIAL: the fold as one thread computes it: every interior
+ *     candidate in row-major order from +inf, then merged with max_dt by the
+ *     combiner (a fold that ends on a NaN leaves max_dt).
+ *   ORC_CFL_DEFINED: the minimum over the candidates that are not NaN, and
+ *     max_dt; independent of order and thread count, and equal to the serial
+ *     fold whenever no candidate is NaN. */
 double orc_set_timestep(int nx, int ny, const double* Qxx, const double* Qyy,
                         const double* rho, const double* e, OrcMesh* m,
                         int first_step) {
   const int pad = m->pad;
+  double local_min;
+  int undefined = 0;
+  if (cfl_rule == ORC_CFL_OMP3_SERIAL) {
+    double fold = INFINITY;
+    for (int i = pad; i < ny - pad; ++i)
+      for (int j = pad; j < nx - pad; ++j) {
+        const double cell_min = cfl_candidate(Qxx, Qyy, rho, e, m, nx, i, j);
+        undefined += (cell_min != cell_min);
+        fold = MIN(fold, cell_min);
+      }
+    local_min = MIN(fold, m->max_dt);
+  } else {
+    local_min = m->max_dt;
+#pragma omp parallel for reduction(min : local_min) reduction(+ : undefined)
+    for (int i = pad; i < ny - pad; ++i)
+      for (int j = pad; j < nx - pad; ++j) {
+        const double cell_min = cfl_candidate(Qxx, Qyy, rho, e, m, nx, i, j);
+        if (cell_min == cell_min)
+          local_min = MIN(local_min, cell_min);
+        else
+          ++undefined;
+      }
+  }
+  last_undefined = undefined;
+  timestep_update(m, local_min, first_step);
+  return local_min;
+}
+
+/* TEST ONLY: omp3/flow.c:77-105 with its loop as the reference writes it, a
+ * min reduction over nthreads OpenMP threads with the ternary min in the
+ * body. With one thread GCC's OpenMP gives ORC_CFL_OMP3_SERIAL; with more,
+ * and NaN candidates, the result depends on the thread count. */
+double orc_set_timestep_literal(int nx, int ny, const double* Qxx,
+                                const double* Qyy, const double* rho,
+                                const double* e, OrcMesh* m, int first_step,
+                                int nthreads) {
+  const int pad = m->pad;
   double local_min = m->max_dt;
-#pragma omp parallel for reduction(min : local_min)
+#pragma omp parallel for num_threads(nthreads) reduction(min : local_min)
   for (int i = pad; i < ny - pad; ++i)
-    for (int j = pad; j < nx - pad; ++j) {
-      const int c = i * nx + j;
-      const double c_s = sqrt(GAM * (GAM - 1.0) * e[c]);
-      const double dtx =
-          m->celldx[j] / sqrt(c_s * c_s + 2.0 * Qxx[c] / rho[c]);
-      const double dty =
-          m->celldy[i] / sqrt(c_s * c_s + 2.0 * Qyy[c] / rho[c]);
-      const double cell_min = MIN(dtx, dty);
-      /* A cell with negative energy gives a NaN candidate (sqrt of a
-       * negative). The reference's min(a, b) = a < b ? a : b drops it unless
-       * it is the last value an OpenMP thread sees, so its effective result
-       * is the minimum over the defined candidates; the oracle takes that
-       * minimum explicitly, independent of the thread count. */
-      if (cell_min == cell_min) local_min = MIN(local_min, cell_min);
-    }
-  const double final_min = MIN(local_min, C_M * m->dt_h);
-  m->dt = 0.5 * (C_T * final_min + m->dt_h);
-  m->dt_h = first_step ? m->dt : C_T * final_min;
+    for (int j = pad; j < nx - pad; ++j)
+      local_min = MIN(local_min, cfl_candidate(Qxx, Qyy, rho, e, m, nx, i, j));
+  timestep_update(m, local_min, first_step);
   return local_min;
 }
 

@@ -70,10 +70,28 @@ void orc_shock_heating_and_work(int nx, int ny, const OrcMesh* m, double dt_h,
                                 double* e, const double* P, const double* u,
                                 const double* v, const double* rho,
                                 const double* Qxx, const double* Qyy);
-/* Returns the local min dt before the clamp, for tests. */
+/* Returns the local min dt before the clamp, for tests. The rule for NaN
+ * candidates (cells with negative energy) is set by orc_set_cfl_rule:
+ * ORC_CFL_DEFINED (the default) takes the minimum over the defined
+ * candidates; ORC_CFL_OMP3_SERIAL folds every candidate in row-major order
+ * with the reference's min(a, b) = a < b ? a : b, as one OpenMP thread does
+ * (a NaN restarts the running minimum), then merges it into max_dt with
+ * OpenMP's min combiner. */
+#define ORC_CFL_DEFINED 0
+#define ORC_CFL_OMP3_SERIAL 1
+void orc_set_cfl_rule(int rule);
+int orc_get_cfl_rule(void);
+/* NaN candidates met by the last orc_set_timestep call. */
+int orc_last_undefined(void);
 double orc_set_timestep(int nx, int ny, const double* Qxx, const double* Qyy,
                         const double* rho, const double* e, OrcMesh* m,
                         int first_step);
+/* TEST ONLY: the reference's loop form, a min reduction over nthreads
+ * OpenMP threads; thread-count dependent when a candidate is NaN. */
+double orc_set_timestep_literal(int nx, int ny, const double* Qxx,
+                                const double* Qyy, const double* rho,
+                                const double* e, OrcMesh* m, int first_step,
+                                int nthreads);
 void orc_x_mass_and_energy_flux(int nx, int ny, int first, const OrcMesh* m,
                                 double dt, double dt_h, double* rho,
                                 double* rho_old, double* e, const double* u,
@@ -113,11 +131,13 @@ void orc_solve_hydro_2d(OrcMesh* m, int tt, double* P, double* rho,
 /* The main.c loop (main.c:64-115) without I/O: pre-loop set_timestep with
  * first_step=0, then niters x (solve + conservation), stopping at sim_end.
  * Records per-step dt (the value printed at the start of each step), mass and
- * energy (rho*e) totals. Returns the number of steps taken.
+ * energy (rho*e) totals, and the NaN CFL candidates of the pre-loop call and
+ * of each step (niters + 1 counts). Returns the number of steps taken.
  * `arrays` holds the 16 state/scratch arrays in solve_hydro_2d order:
  * P rho rho_old e u v mom_x mom_y Qxx Qyy F_x F_y uF_x uF_y vF_x vF_y. */
 int orc_run(OrcMesh* m, double** arrays, int niters, double* dt_seq,
-            double* mass_seq, double* energy_seq, double* wall_seconds);
+            double* mass_seq, double* energy_seq, double* wall_seconds,
+            int* undefined_seq);
 
 #ifdef __cplusplus
 }

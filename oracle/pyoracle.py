@@ -56,6 +56,11 @@ def lib():
         L.orc_shock_heating_and_work.argtypes = [i, i, mp, d] + [vp] * 7
         L.orc_set_timestep.restype = d
         L.orc_set_timestep.argtypes = [i, i, vp, vp, vp, vp, mp, i]
+        L.orc_set_timestep_literal.restype = d
+        L.orc_set_timestep_literal.argtypes = [i, i, vp, vp, vp, vp, mp, i, i]
+        L.orc_set_cfl_rule.argtypes = [i]
+        L.orc_get_cfl_rule.restype = i
+        L.orc_last_undefined.restype = i
         L.orc_x_mass_and_energy_flux.argtypes = [i, i, i, mp, d, d] + [vp] * 6
         L.orc_y_mass_and_energy_flux.argtypes = [i, i, i, mp, d, d] + [vp] * 6
         L.orc_advect_mass_and_energy.argtypes = [i, i, mp, i, d, d] + [vp] * 9
@@ -67,7 +72,7 @@ def lib():
         L.orc_conservation.argtypes = [i, i, i, vp, vp, _dp, _dp]
         L.orc_solve_hydro_2d.argtypes = [mp, i] + [vp] * 16
         L.orc_run.restype = i
-        L.orc_run.argtypes = [mp, ctypes.POINTER(vp), i, vp, vp, vp, _dp]
+        L.orc_run.argtypes = [mp, ctypes.POINTER(vp), i, vp, vp, vp, _dp, vp]
         _lib = L
     return _lib
 
@@ -144,6 +149,41 @@ def set_timestep(mesh, arrays, first_step):
                                   ptr(arrays["e"]), mesh.ref, first_step)
 
 
+# Rules for NaN CFL candidates (oracle.h): the minimum over the defined
+# candidates, or omp3's fold as one OpenMP thread computes it.
+CFL_DEFINED, CFL_OMP3_SERIAL = 0, 1
+
+
+def set_cfl_rule(rule):
+    lib().orc_set_cfl_rule(rule)
+
+
+def get_cfl_rule():
+    return lib().orc_get_cfl_rule()
+
+
+class cfl_rule:
+    """Context manager: run the oracle under one CFL rule, then restore."""
+
+    def __init__(self, rule):
+        self.rule = rule
+
+    def __enter__(self):
+        self.prev = get_cfl_rule()
+        set_cfl_rule(self.rule)
+        return self
+
+    def __exit__(self, *exc):
+        set_cfl_rule(self.prev)
+
+
+def set_timestep_literal(mesh, arrays, first_step, nthreads):
+    """omp3's loop form (a min reduction over nthreads threads); test only."""
+    return lib().orc_set_timestep_literal(
+        mesh.nx, mesh.ny, ptr(arrays["Qxx"]), ptr(arrays["Qyy"]),
+        ptr(arrays["rho"]), ptr(arrays["e"]), mesh.ref, first_step, nthreads)
+
+
 def conservation(mesh, arrays):
     m, e = ctypes.c_double(), ctypes.c_double()
     lib().orc_conservation(mesh.nx, mesh.ny, mesh.pad, ptr(arrays["rho"]),
@@ -151,14 +191,21 @@ def conservation(mesh, arrays):
     return m.value, e.value
 
 
-def run(mesh, arrays, niters):
-    """main.c:64-115 without I/O. Returns (steps, dt_pairs, mass, energy, wall_s)."""
+def run(mesh, arrays, niters, undefined=None):
+    """main.c:64-115 without I/O. Returns (steps, dt_pairs, mass, energy,
+    wall_s). undefined, if given, is an int32 array of niters + 1 that
+    receives the NaN CFL candidates of the pre-loop call and of each step."""
     dt_seq = np.zeros(2 * (niters + 1))
     mass = np.zeros(niters)
     energy = np.zeros(niters)
     wall = ctypes.c_double()
     ptrs = (ctypes.c_void_p * 16)(*[arrays[k].ctypes.data for k in ARRAY_NAMES])
+    if undefined is not None:
+        assert undefined.dtype == np.int32 and undefined.size >= niters + 1
+        und = undefined.ctypes.data_as(ctypes.c_void_p)
+    else:
+        und = None
     steps = lib().orc_run(mesh.ref, ptrs, niters, ptr(dt_seq), ptr(mass),
-                          ptr(energy), ctypes.byref(wall))
+                          ptr(energy), ctypes.byref(wall), und)
     return (steps, dt_seq[: 2 * (steps + 1)].reshape(-1, 2), mass[:steps],
             energy[:steps], wall.value)

@@ -14,7 +14,7 @@ def sides(n):
 
 
 def oracle_run(problem_fn, n, steps, width=1.0, max_dt=1.0e30, dt=1.0e-4,
-               init=None):
+               init=None, cfl_rule=0):
     from oracle import pyoracle as po
     nx, ny = sides(n)
     m = po.Mesh(nx, ny, width=width, height=width, max_dt=max_dt, dt=dt)
@@ -23,17 +23,21 @@ def oracle_run(problem_fn, n, steps, width=1.0, max_dt=1.0e30, dt=1.0e-4,
     a["rho"][:] = rho
     a["e"][:] = e
     po.reflect_initial(m, a)
-    steps_done, dts, mass, energy, wall = po.run(m, a, steps)
+    und = np.zeros(steps + 1, dtype=np.int32)
+    with po.cfl_rule(cfl_rule):
+        steps_done, dts, mass, energy, wall = po.run(m, a, steps, undefined=und)
     return dict(arrays=a, dt=dts, mass=mass, energy=energy, steps=steps_done,
-                L=m.nx if nx == ny else (m.nx, m.ny), wall=wall)
+                L=m.nx if nx == ny else (m.nx, m.ny), wall=wall,
+                undefined=und[: steps_done + 1])
 
 
 def gpu_run(problem_fn, n, steps, schedule, width=1.0, max_dt=1.0e30,
-            dt=1.0e-4, graphs=False, init=None, march_rows=None):
+            dt=1.0e-4, graphs=False, init=None, march_rows=None, cfl_rule=0):
     import flow_amd
     nx, ny = sides(n)
     m = flow_amd.Mesh(nx, ny, width=width, height=width, max_dt=max_dt, dt=dt)
     m.set_schedule(schedule)
+    m.set_cfl_rule(cfl_rule)
     if march_rows:
         m.set_march_rows(march_rows)
     m.set_graphs(graphs)
@@ -44,7 +48,8 @@ def gpu_run(problem_fn, n, steps, schedule, width=1.0, max_dt=1.0e30,
     st.reflect_initial()
     steps_done, dts, mass, energy, wall = st.run(steps)
     out = dict(arrays=st.download_all(), dt=dts, mass=mass, energy=energy,
-               steps=steps_done, L=m.nx if nx == ny else (m.nx, m.ny), wall=wall)
+               steps=steps_done, L=m.nx if nx == ny else (m.nx, m.ny), wall=wall,
+               undefined=st.undefined)
     st.close()
     m.close()
     return out
@@ -98,13 +103,23 @@ def rel_linf(a, b):
     return float(num / den)
 
 
-def compare_state(got, ref, L, fields=("rho", "e", "u", "v", "mom_x", "mom_y")):
+def compare_state(got, ref, L, fields=("rho", "e", "u", "v", "mom_x", "mom_y"),
+                  nan_equal=False):
     """Per field: (relative L-inf over the defined region, number of
-    elements that differ bitwise)."""
+    elements that differ bitwise). With nan_equal, a NaN on both sides
+    counts as equal (NaN payloads are not compared) and the L-inf is taken
+    over the elements finite on both sides."""
     g, r = views(got, L), views(ref, L)
     reg = state_regions(L)
     out = {}
     for k in fields:
         a, b = g[k][reg[k]], r[k][reg[k]]
-        out[k] = (rel_linf(a, b), int(np.count_nonzero(a != b)))
+        if nan_equal:
+            both = np.isnan(a) & np.isnan(b)
+            ndiff = int(np.count_nonzero((a != b) & ~both))
+            fin = np.isfinite(a) & np.isfinite(b)
+            rel = rel_linf(a[fin], b[fin]) if fin.any() else 0.0
+            out[k] = (rel, ndiff)
+        else:
+            out[k] = (rel_linf(a, b), int(np.count_nonzero(a != b)))
     return out

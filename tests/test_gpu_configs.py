@@ -7,6 +7,13 @@
   properties that hold at any size: interior mass conserved to round-off
   (reflective walls pass no flux), every field and dt finite and positive
   where it must be, and two runs bitwise identical.
+
+The 4096^2 blast's first step runs at 0.87 of the CFL limit (the .params dt
+of 1e-4 blended into the pre-loop dt), and shock heating drives cells in and
+around the blast to negative energy: their CFL candidates are undefined in
+every step. These runs use the default rule (the minimum over the defined
+candidates) and report the count each step; the same run under omp3's
+serial rule, which diverges, is in test_gpu_cfl_rules.py.
 """
 import numpy as np
 import pytest
@@ -44,6 +51,10 @@ def test_blast_4096_first_20_steps_bitwise():
     print("case blast_4096 n=4096 steps=20 march", flush=True)
     ref = harness.oracle_run(problems.blast_4096, 4096, 20, dt=1e-4)
     got = harness.gpu_run(problems.blast_4096, 4096, 20, flow_amd.MARCH, dt=1e-4)
+    print(f"  undefined CFL candidates per step: {got['undefined'].tolist()}",
+          flush=True)
+    np.testing.assert_array_equal(got["undefined"], ref["undefined"])
+    assert got["undefined"][0] == 0 and got["undefined"][1] > 0
     _bitwise(got, ref, sum_rtol=1e-12)
 
 
@@ -63,6 +74,15 @@ def test_blast_4096_500_steps_invariants():
     for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
         assert np.all(np.isfinite(v[k][reg[k]])), k
     assert np.all(v["rho"][2:L - 2, 2:L - 2] > 0)
+    und = a["undefined"]
+    e_int = v["e"][2:L - 2, 2:L - 2]
+    print(f"  undefined CFL candidates: {int((und > 0).sum())} of 500 steps, "
+          f"at most {int(und.max())} cells; min e after 500 steps "
+          f"{float(e_int.min()):.6g} ({int((e_int < 0).sum())} cells below 0)",
+          flush=True)
+    # the defined rule runs on: every step's dt is the minimum over the cells
+    # whose candidate is defined (a negative-energy cell has none)
+    assert und[1] > 0
     b = harness.gpu_run(problems.blast_4096, 4096, 500, flow_amd.MARCH, dt=1e-4)
     for k, (rel, ndiff) in harness.compare_state(a["arrays"], b["arrays"], L).items():
         assert ndiff == 0, (k, ndiff)
