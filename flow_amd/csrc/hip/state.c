@@ -3,8 +3,9 @@
  *
  * A state file holds what one timestep carries to the next: the cell fields
  * rho and e, the face velocities u and v, the momenta mom_x and mom_y (each
- * at its layout's exact extent, C: L x L, X: L x (L+1), Y: (L+1) x L with
- * L = n + 2*pad), the timestep pair (dt, dt_h) the next solve_hydro_2d call
+ * at its layout's exact extent in rows x columns, C: Ly x Lx, X: Ly x
+ * (Lx+1), Y: (Ly+1) x Lx with Lx = nx + 2*pad, Ly = ny + 2*pad), the
+ * timestep pair (dt, dt_h) the next solve_hydro_2d call
  * starts from, the number of steps completed and the simulated time. It is
  * the on-disk form of a workload's initial state (step 0) and a restart
  * point (step > 0); flow_amd/statefile.py reads and writes the same format
@@ -36,11 +37,13 @@
 static const char* const kNames[STATE_NARRAYS] = {"rho", "e", "u", "v", "mom_x", "mom_y"};
 static const char kLayouts[STATE_NARRAYS] = {'C', 'C', 'X', 'Y', 'X', 'Y'};
 
-static size_t layout_count(char layout, int L) {
+/* Elements of one array at its layout's extent; rows are contiguous at the
+ * layout's stride, so this is also the prefix of the device array. */
+static size_t layout_count(char layout, int Lx, int Ly) {
   switch (layout) {
-    case 'C': return (size_t)L * L;
-    case 'X': return (size_t)L * (L + 1);
-    default: return (size_t)(L + 1) * L;
+    case 'C': return (size_t)Ly * Lx;
+    case 'X': return (size_t)Ly * (Lx + 1);
+    default: return (size_t)(Ly + 1) * Lx;
   }
 }
 
@@ -69,8 +72,9 @@ static int read_header(FILE* f, const char* path, StateHeader* h) {
   if (memcmp(h->magic, STATE_MAGIC, 8) != 0) return fail(f, path, "not a state file");
   if (h->version != STATE_VERSION) return fail(f, path, "unsupported version");
   if (h->narrays != STATE_NARRAYS) return fail(f, path, "unexpected array count");
-  if (h->global_nx < 1 || h->global_ny < 1 || h->global_nx != h->global_ny || h->pad != 2)
-    return fail(f, path, "unsupported mesh (square interiors with pad 2 only)");
+  if (!flowhip_mesh_size_ok(h->global_nx, h->global_ny) || h->pad != 2)
+    return fail(f, path, "unsupported mesh (2 to FLOWHIP_MAX_CELLS_SIDE interior "
+                         "cells a side, pad 2)");
   return 0;
 }
 
@@ -92,8 +96,7 @@ int flowhip_state_save(const char* path, const Mesh* mesh, int step, double sim_
                        const double* rho, const double* e, const double* u,
                        const double* v, const double* mom_x, const double* mom_y) {
   const double* dev[STATE_NARRAYS] = {rho, e, u, v, mom_x, mom_y};
-  const int L = mesh->local_nx;
-  if (mesh->local_ny != L) return fail(NULL, path, "square meshes only");
+  const int Lx = mesh->local_nx, Ly = mesh->local_ny;
   FILE* f = fopen(path, "wb");
   if (!f) return fail(NULL, path, "cannot create");
   StateHeader h;
@@ -118,7 +121,7 @@ int flowhip_state_save(const char* path, const Mesh* mesh, int step, double sim_
     memset(&a, 0, sizeof(a));
     strncpy(a.name, kNames[k], sizeof(a.name) - 1);
     a.layout = kLayouts[k];
-    a.count = layout_count(a.layout, L);
+    a.count = layout_count(a.layout, Lx, Ly);
     if (a.count > cap) {
       free(host);
       cap = a.count;
@@ -154,7 +157,7 @@ int flowhip_state_load(const char* path, Mesh* mesh, int* step, double* sim_time
     return fail(f, path, "mesh size differs from the run's");
   if (h.width != mesh->width || h.height != mesh->height)
     return fail(f, path, "domain size differs from the run's");
-  const int L = mesh->local_nx;
+  const int Lx = mesh->local_nx, Ly = mesh->local_ny;
   double* host = NULL;
   size_t cap = 0;
   for (int k = 0; k < STATE_NARRAYS; ++k) {
@@ -164,7 +167,7 @@ int flowhip_state_load(const char* path, Mesh* mesh, int* step, double* sim_time
       return fail(f, path, "truncated array header");
     }
     if (strncmp(a.name, kNames[k], sizeof(a.name)) != 0 || a.layout != kLayouts[k] ||
-        a.count != layout_count(kLayouts[k], L)) {
+        a.count != layout_count(kLayouts[k], Lx, Ly)) {
       free(host);
       return fail(f, path, "unexpected array record");
     }

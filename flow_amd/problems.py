@@ -77,9 +77,11 @@ def blast_4096(n):
 
 
 def flow_params_problem(n):
-    """The problem of the reference's flow.params (flow.params:1-2)."""
-    return boxes(n, n, [(0.125, 2.0, 0.0, 0.0, 1.0, 1.0),
-                        (1.0, 2.5, 0.4, 0.4, 0.2, 0.2)])
+    """The problem of the reference's flow.params (flow.params:1-2) on n x n
+    interior cells, or nx x ny for n = (nx, ny)."""
+    nx, ny = (n, n) if np.isscalar(n) else (int(n[0]), int(n[1]))
+    return boxes(nx, ny, [(0.125, 2.0, 0.0, 0.0, 1.0, 1.0),
+                          (1.0, 2.5, 0.4, 0.4, 0.2, 0.2)])
 
 
 def uniform(n, rho0=1.0, e0=2.5):

@@ -1,8 +1,9 @@
 """Bit-exact state files: the numpy side of flow_amd/csrc/hip/state.c.
 
 A state file holds what one timestep carries to the next: rho and e (cell
-layout, L x L), u and mom_x (x-face layout, L x (L+1)), v and mom_y (y-face
-layout, (L+1) x L), with L = n + 2*pad; the timestep pair (dt, dt_h) the next
+layout, Ly x Lx), u and mom_x (x-face layout, Ly x (Lx+1)), v and mom_y
+(y-face layout, (Ly+1) x Lx), with Lx = nx + 2*pad and Ly = ny + 2*pad
+(rows x columns); the timestep pair (dt, dt_h) the next
 step starts from; the steps completed and the simulated time. Step 0 is an
 initial state, a later step a restart point. The driver (flow.hip --dump /
 --load), the library (flowhip_state_save / _load) and this module read and
@@ -27,23 +28,36 @@ _HEADER = struct.Struct("<8sII4i5d")
 _RECORD = struct.Struct("<16sc7sQ")
 
 
-def layout_shape(layout, L):
-    """Rows and columns of a layout for a mesh of L = n + 2*pad."""
-    return {"C": (L, L), "X": (L, L + 1), "Y": (L + 1, L)}[layout]
+# Largest interior side the library indexes (FLOWHIP_MAX_CELLS_SIDE).
+MAX_SIDE = 46335
+
+
+def sides(n):
+    """(nx, ny) interior cells of a mesh given as n (square) or (nx, ny)."""
+    return (int(n), int(n)) if np.isscalar(n) else (int(n[0]), int(n[1]))
+
+
+def layout_shape(layout, Lx, Ly=None):
+    """Rows and columns of a layout for a mesh of Lx = nx + 2*pad columns of
+    cells and Ly = ny + 2*pad rows (Ly defaults to Lx)."""
+    Ly = Lx if Ly is None else Ly
+    return {"C": (Ly, Lx), "X": (Ly, Lx + 1), "Y": (Ly + 1, Lx)}[layout]
 
 
 def write(path, arrays, n, step=0, width=1.0, height=1.0, dt=0.0, dt_h=0.0,
           sim_time=0.0):
-    """Write a state file. arrays maps each carried field name to either a
-    2-D array of its layout's shape or a flat buffer holding at least that
-    many values (row-major, as the library's (L+1)^2 allocations do)."""
-    L = n + 2 * PAD
+    """Write a state file for a mesh of n (square) or (nx, ny) interior
+    cells. arrays maps each carried field name to either a 2-D array of its
+    layout's shape or a flat buffer holding at least that many values
+    (row-major, as the library's (Lx+1)(Ly+1) allocations do)."""
+    nx, ny = sides(n)
+    Lx, Ly = nx + 2 * PAD, ny + 2 * PAD
     with open(path, "wb") as f:
-        f.write(_HEADER.pack(MAGIC, VERSION, len(ARRAYS), n, n, PAD, int(step),
+        f.write(_HEADER.pack(MAGIC, VERSION, len(ARRAYS), nx, ny, PAD, int(step),
                              float(width), float(height), float(dt), float(dt_h),
                              float(sim_time)))
         for name, layout in ARRAYS:
-            rows, cols = layout_shape(layout, L)
+            rows, cols = layout_shape(layout, Lx, Ly)
             a = np.ascontiguousarray(np.asarray(arrays[name], dtype="<f8").reshape(-1))
             if a.size < rows * cols:
                 raise ValueError(f"{name}: {a.size} values, layout needs {rows * cols}")
@@ -52,8 +66,8 @@ def write(path, arrays, n, step=0, width=1.0, height=1.0, dt=0.0, dt_h=0.0,
 
 
 def read(path):
-    """Read a state file: a dict with the header fields and 'arrays', each a
-    2-D float64 array of its layout's shape."""
+    """Read a state file: a dict with the header fields (n is nx) and
+    'arrays', each a 2-D float64 array of its layout's shape."""
     with open(path, "rb") as f:
         head = f.read(_HEADER.size)
         if len(head) != _HEADER.size:
@@ -62,16 +76,17 @@ def read(path):
          sim_time) = _HEADER.unpack(head)
         if magic != MAGIC:
             raise ValueError(f"{path}: not a state file")
-        if version != VERSION or narrays != len(ARRAYS) or pad != PAD or nx != ny:
+        if (version != VERSION or narrays != len(ARRAYS) or pad != PAD
+                or not (2 <= nx <= MAX_SIDE and 2 <= ny <= MAX_SIDE)):
             raise ValueError(f"{path}: unsupported state file")
-        L = nx + 2 * pad
+        Lx, Ly = nx + 2 * pad, ny + 2 * pad
         arrays = {}
         for name, layout in ARRAYS:
             rec = f.read(_RECORD.size)
             if len(rec) != _RECORD.size:
                 raise ValueError(f"{path}: truncated record")
             rname, rlayout, _, count = _RECORD.unpack(rec)
-            rows, cols = layout_shape(layout, L)
+            rows, cols = layout_shape(layout, Lx, Ly)
             if (rname.rstrip(b"\0").decode() != name or rlayout.decode() != layout
                     or count != rows * cols):
                 raise ValueError(f"{path}: unexpected record for {name}")
@@ -79,17 +94,19 @@ def read(path):
             if data.size != count:
                 raise ValueError(f"{path}: truncated data for {name}")
             arrays[name] = data.reshape(rows, cols).astype(np.float64)
-    return dict(n=nx, pad=pad, step=step, width=width, height=height, dt=dt,
-                dt_h=dt_h, sim_time=sim_time, arrays=arrays)
+    return dict(n=nx, nx=nx, ny=ny, pad=pad, step=step, width=width, height=height,
+                dt=dt, dt_h=dt_h, sim_time=sim_time, arrays=arrays)
 
 
 def initial_state(n, rho, e):
-    """Carried fields of an initial state from flat (L+1)^2 rho and e
-    buffers (the problems module's output): velocities and momenta zero."""
-    L = n + 2 * PAD
+    """Carried fields of an initial state from flat (Lx+1)(Ly+1) rho and e
+    buffers (the problems module's output) of a mesh of n or (nx, ny)
+    interior cells: velocities and momenta zero."""
+    nx, ny = sides(n)
+    Lx, Ly = nx + 2 * PAD, ny + 2 * PAD
     out = {}
     for name, layout in ARRAYS:
-        rows, cols = layout_shape(layout, L)
+        rows, cols = layout_shape(layout, Lx, Ly)
         src = {"rho": rho, "e": e}.get(name)
         out[name] = (np.asarray(src, dtype=np.float64)[: rows * cols].reshape(rows, cols)
                      if src is not None else np.zeros((rows, cols)))

@@ -22,15 +22,17 @@ KAT = json.load(open(os.path.join(HERE, "golden", "flow_tests_kat.json")))
 
 
 def _case(tmp_path, n, iterations, dt, boxes, tests=None, visit_dump=0,
-          sim_end=1.0e30):
-    """A run directory with flow.params (and flow.tests) below arch.params."""
+          sim_end=1.0e30, ny=None):
+    """A run directory with flow.params (and flow.tests) below arch.params;
+    n x n interior cells, or n x ny."""
     (tmp_path / "arch.params").write_text(
         f"width 1.0\nheight 1.0\nmax_dt 1.0e30\nsim_end {sim_end!r}\n")
     run = tmp_path / "flow"
     run.mkdir(exist_ok=True)
     lines = [f"problem_{k} density={b[0]!r} energy={b[1]!r} xpos={b[2]!r} "
              f"ypos={b[3]!r} width={b[4]!r} height={b[5]!r}" for k, b in enumerate(boxes)]
-    lines += [f"dt {dt!r}", f"nx {n}", f"ny {n}", f"iterations {iterations}",
+    lines += [f"dt {dt!r}", f"nx {n}", f"ny {n if ny is None else ny}",
+              f"iterations {iterations}",
               f"visit_dump {visit_dump}"]
     (run / "flow.params").write_text("\n".join(lines) + "\n")
     if tests:
@@ -105,6 +107,33 @@ def test_driver_restart_continues_bit_for_bit(tmp_path):
     reg = harness.state_regions(L)
     for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
         assert a["arrays"][k][reg[k]].tobytes() == b["arrays"][k][reg[k]].tobytes(), k
+
+
+def test_driver_rectangular_dump_and_restart(tmp_path):
+    """A 64 x 40 mesh (nx != ny): --dump equals the oracle's state, and a
+    restart from step 3 plus 3 steps equals 6 straight steps bit for bit."""
+    nx, ny, dt = 64, 40, 1e-4
+    print(f"case flow.hip rectangular {nx}x{ny} dump and restart", flush=True)
+    (tmp_path / "whole").mkdir()
+    whole = _case(tmp_path / "whole", nx, 6, dt, KAT["problems"], ny=ny)
+    rc, out, err = _drive(whole, "--dump", "six.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    (tmp_path / "first").mkdir()
+    first = _case(tmp_path / "first", nx, 3, dt, KAT["problems"], ny=ny)
+    rc, out, err = _drive(first, "--dump", "three.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    rc, out, err = _drive(first, "--load", "three.state", "--dump", "resumed.state")
+    assert rc == 0, out[-2000:] + err[-2000:]
+    a, b = sf.read(first / "resumed.state"), sf.read(whole / "six.state")
+    assert (a["nx"], a["ny"], a["step"]) == (nx, ny, 6)
+    assert (a["dt"], a["dt_h"], a["sim_time"]) == (b["dt"], b["dt_h"], b["sim_time"])
+    L = (nx + 4, ny + 4)
+    reg = harness.state_regions(L)
+    ref = harness.oracle_run(problems.flow_params_problem, (nx, ny), 6, dt=dt)
+    v = harness.views(ref["arrays"], L)
+    for k in ("rho", "e", "u", "v", "mom_x", "mom_y"):
+        assert a["arrays"][k][reg[k]].tobytes() == b["arrays"][k][reg[k]].tobytes(), k
+        assert np.count_nonzero(b["arrays"][k][reg[k]] != v[k][reg[k]]) == 0, k
 
 
 def test_driver_runs_a_generated_initial_state(tmp_path):

@@ -61,13 +61,35 @@ def test_initial_state_from_problem_buffers():
     assert st["mom_y"].shape == (L + 1, L) and not st["mom_y"].any()
 
 
+def test_rectangular_round_trip_and_shapes(tmp_path):
+    """A 64 x 40 mesh: Lx = 68 columns, Ly = 44 rows; x-face arrays 44 x 69,
+    y-face arrays 45 x 68; the header holds nx then ny."""
+    rng = np.random.default_rng(11)
+    Lx, Ly = 68, 44
+    st = {name: rng.standard_normal(sf.layout_shape(lay, Lx, Ly)) for name, lay in sf.ARRAYS}
+    assert st["u"].shape == (44, 69) and st["v"].shape == (45, 68)
+    p = tmp_path / "r.bin"
+    sf.write(p, st, (64, 40), step=3)
+    b = p.read_bytes()
+    assert struct.unpack_from("<4i", b, 16) == (64, 40, 2, 3)
+    d = sf.read(p)
+    assert (d["nx"], d["ny"]) == (64, 40)
+    for name, _ in sf.ARRAYS:
+        assert d["arrays"][name].tobytes() == st[name].tobytes(), name
+    cells, xf, yf = Lx * Ly, Ly * (Lx + 1), (Ly + 1) * Lx
+    assert len(b) == 72 + 6 * 32 + 8 * (2 * cells + 2 * xf + 2 * yf)
+
+
 def test_malformed_files_are_refused(tmp_path):
     p = tmp_path / "s.bin"
     sf.write(p, _state(2), 2)
     good = p.read_bytes()
     (tmp_path / "bad_magic.bin").write_bytes(b"XLOWSTAT" + good[8:])
     (tmp_path / "short.bin").write_bytes(good[:-8])
-    for bad in ("bad_magic.bin", "short.bin"):
+    huge = bytearray(good)
+    struct.pack_into("<2i", huge, 16, 1 << 30, 2)
+    (tmp_path / "huge.bin").write_bytes(bytes(huge))
+    for bad in ("bad_magic.bin", "short.bin", "huge.bin"):
         with pytest.raises(ValueError):
             sf.read(tmp_path / bad)
 
