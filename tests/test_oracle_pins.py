@@ -373,6 +373,167 @@ def test_momentum_flux_uniform_and_linear(oracle):
     assert as2d(a["uF_x"], L, L)[i, j] == pytest.approx(want, rel=1e-13)
 
 
+def _mom_flux(po, fn, m, dt_h, *arrs):
+    getattr(po.lib(), fn)(m.nx, m.ny, m.ref, dt_h, *[po.ptr(x) for x in arrs])
+
+
+def test_momentum_x_flux_in_y_closed_form(oracle):
+    """uF_y at corner (i, j), omp3/flow.c:826-874. Uniform u: F u exactly,
+    with F = dx (F_y[i, j] + F_y[i, j-1]) / 2 (F_y varies along x, so a
+    shifted average fails). u linear in y with slope s: the MC slope is s,
+    and the corner value is the upwind face value moved by half a cell
+    against the corner velocity v_cell = (v[i, j-1] + v[i, j]) / 2: for
+    v_cell > 0, u[i-1] + s/2 (dy - v_cell dt_h); otherwise u[i] - s/2 (dy +
+    v_cell dt_h). v varies along x and changes sign, so both branches and
+    the v_cell columns are tested."""
+    po = oracle
+    n = 16
+    h = 1.0 / n
+    m, a = grid_arrays(po, n)
+    L = m.nx
+    x = (np.arange(L) - PAD + 0.5) * h
+    Fy = as2d(a["F_y"], L + 1, L)
+    Fy[:] = (0.6 + 0.9 * x)[None, :]
+    V = as2d(a["v"], L + 1, L)
+    V[:] = (-0.45 + 1.1 * x)[None, :]
+    U = as2d(a["u"], L, L + 1)
+    U[:] = 0.35
+    dt_h = 2e-2
+    _mom_flux(po, "orc_momentum_x_flux_in_y", m, dt_h, a["u"], a["v"], a["uF_y"], a["F_y"])
+    uF = as2d(a["uF_y"], L + 1, L + 1)
+    for i, j in ((5, 3), (9, 14)):
+        F = h * 0.5 * (Fy[i, j] + Fy[i, j - 1])
+        assert uF[i, j] == pytest.approx(F * 0.35, rel=1e-14)
+    s = 1.7
+    yf = (np.arange(L) - PAD + 0.5) * h
+    U[:] = (0.2 + s * yf)[:, None]
+    _mom_flux(po, "orc_momentum_x_flux_in_y", m, dt_h, a["u"], a["v"], a["uF_y"], a["F_y"])
+    for i, j in ((6, 3), (7, 4), (8, 13), (11, 15)):
+        vc = 0.5 * (V[i, j - 1] + V[i, j])
+        F = h * 0.5 * (Fy[i, j] + Fy[i, j - 1])
+        if vc > 0.0:
+            corner = U[i - 1, j] + 0.5 * s * (h - vc * dt_h)
+        else:
+            corner = U[i, j] - 0.5 * s * (h + vc * dt_h)
+        assert uF[i, j] == pytest.approx(F * corner, rel=1e-13), (i, j, vc)
+    assert (V[6, 2] + V[6, 3]) < 0 < (V[8, 12] + V[8, 13])
+
+
+def test_momentum_y_flux_in_x_closed_form(oracle):
+    """vF_x at corner (i, j), omp3/flow.c:877-925. Uniform v: F v exactly,
+    F = dy (F_x[i, j] + F_x[i-1, j]) / 2 (F_x varies along y). v linear in x
+    with slope s: for u_cell = (u[i, j] + u[i-1, j]) / 2 > 0 the value is
+    v[j-1] + s/2 (dx - u_cell dt_h), otherwise v[j] - s/2 (dx + u_cell dt_h);
+    u varies along y and changes sign."""
+    po = oracle
+    n = 16
+    h = 1.0 / n
+    m, a = grid_arrays(po, n)
+    L = m.nx
+    y = (np.arange(L) - PAD + 0.5) * h
+    Fx = as2d(a["F_x"], L, L + 1)
+    Fx[:] = (0.5 + 1.3 * y)[:, None]
+    U = as2d(a["u"], L, L + 1)
+    U[:] = (-0.5 + 1.2 * y)[:, None]
+    V = as2d(a["v"], L + 1, L)
+    V[:] = -0.25
+    dt_h = 2e-2
+    _mom_flux(po, "orc_momentum_y_flux_in_x", m, dt_h, a["u"], a["v"], a["vF_x"], a["F_x"])
+    vF = as2d(a["vF_x"], L + 1, L + 1)
+    for i, j in ((4, 6), (13, 9)):
+        F = h * 0.5 * (Fx[i, j] + Fx[i - 1, j])
+        assert vF[i, j] == pytest.approx(F * -0.25, rel=1e-14)
+    s = -2.1
+    xc = (np.arange(L) - PAD + 0.5) * h
+    V[:] = (0.3 + s * xc)[None, :]
+    _mom_flux(po, "orc_momentum_y_flux_in_x", m, dt_h, a["u"], a["v"], a["vF_x"], a["F_x"])
+    for i, j in ((3, 5), (6, 8), (12, 7), (15, 10)):
+        uc = 0.5 * (U[i, j] + U[i - 1, j])
+        F = h * 0.5 * (Fx[i, j] + Fx[i - 1, j])
+        if uc > 0.0:
+            val = V[i, j - 1] + 0.5 * s * (h - uc * dt_h)
+        else:
+            val = V[i, j] - 0.5 * s * (h + uc * dt_h)
+        assert vF[i, j] == pytest.approx(F * val, rel=1e-13), (i, j, uc)
+    assert (U[3, 5] + U[2, 5]) < 0 < (U[12, 7] + U[11, 7])
+
+
+def test_momentum_y_flux_in_y_closed_form(oracle):
+    """vF_y at cell (i, j), omp3/flow.c:928-970. Uniform v: F v exactly,
+    F = dx (F_y[i, j] + F_y[i+1, j]) / 2 (F_y varies along y). v linear in y
+    with slope s: for v_cell = (v[i] + v[i+1]) / 2 > 0 the value is
+    v[i] + s/2 (dy - v_cell dt_h), otherwise v[i+1] - s/2 (dy + v_cell dt_h)."""
+    po = oracle
+    n = 16
+    h = 1.0 / n
+    m, a = grid_arrays(po, n)
+    L = m.nx
+    yf = (np.arange(L + 1) - PAD) * h
+    Fy = as2d(a["F_y"], L + 1, L)
+    Fy[:] = (0.7 + 0.8 * yf)[:, None]
+    V = as2d(a["v"], L + 1, L)
+    V[:] = 0.45
+    dt_h = 2e-2
+    _mom_flux(po, "orc_momentum_y_flux_in_y", m, dt_h, a["v"], a["vF_y"], a["F_y"])
+    vF = as2d(a["vF_y"], L, L)
+    for i, j in ((5, 5), (12, 3)):
+        F = h * 0.5 * (Fy[i, j] + Fy[i + 1, j])
+        assert vF[i, j] == pytest.approx(F * 0.45, rel=1e-14)
+    s = 1.4
+    V[:] = (-0.6 + s * yf)[:, None]
+    _mom_flux(po, "orc_momentum_y_flux_in_y", m, dt_h, a["v"], a["vF_y"], a["F_y"])
+    for i, j in ((3, 4), (6, 9), (12, 10), (16, 3)):
+        vc = 0.5 * (V[i, j] + V[i + 1, j])
+        F = h * 0.5 * (Fy[i, j] + Fy[i + 1, j])
+        if vc > 0.0:
+            val = V[i, j] + 0.5 * s * (h - vc * dt_h)
+        else:
+            val = V[i + 1, j] - 0.5 * s * (h + vc * dt_h)
+        assert vF[i, j] == pytest.approx(F * val, rel=1e-13), (i, j, vc)
+    assert V[3, 4] + V[4, 4] < 0 < V[12, 10] + V[13, 10]
+
+
+def test_y_flux_limiter_follows_v_not_v_tc(oracle):
+    """omp3/flow.c:477: the y sweep picks the Van Leer ratio by the sign of
+    the face velocity v, where the x sweep uses u_tc (:361). With v linear in
+    y (slope 2) and dt = 2, v_tc = v (1 - dt s / 2) = -v: at a face with v > 0
+    the upwind density is the upper cell (v_tc < 0) while the ratio is the
+    lower one's, r = (rho[i-1] - rho[i-2]) / (rho[i] - rho[i-1]). Densities
+    1.0, 1.1, 1.3, 1.9 make the lower and upper ratios 0.5 and 3."""
+    po = oracle
+    n = 16
+    h = 1.0 / n
+    m, a = grid_arrays(po, n)
+    L = m.nx
+    i = 8
+    R = as2d(a["rho"], L, L)
+    R[:] = 1.0
+    R[i - 2], R[i - 1], R[i], R[i + 1] = 1.0, 1.1, 1.3, 1.9
+    R[i + 2:] = 1.9
+    as2d(a["e"], L, L)[:] = 1.0
+    a["rho_old"][:] = a["rho"]
+    yf = (np.arange(L + 1) - PAD) * h
+    V = as2d(a["v"], L + 1, L)
+    V[:] = (0.05 - 2.0 * yf[i] + 2.0 * yf)[:, None]   # v = 0.05 at face i
+    dt, dt_h = 2.0, 1e-3
+    po.lib().orc_y_mass_and_energy_flux(
+        L, L, 1, m.ref, dt, dt_h, po.ptr(a["rho"]), po.ptr(a["rho_old"]),
+        po.ptr(a["e"]), po.ptr(a["v"]), po.ptr(a["F_y"]), po.ptr(a["uF_y"]))
+    F = as2d(a["F_y"], L + 1, L)
+    v = V[i, 5]
+    v_tc = v - 0.5 * v * dt * 2.0
+    assert v > 0 > v_tc
+    drho = 1.3 - 1.1
+    r = (1.1 - 1.0) / drho
+    lim = (r + abs(r)) / (1.0 + abs(r))
+    want = v_tc * 1.3 + 0.5 * abs(v_tc) * (1.0 - abs(v_tc * dt_h / h)) * lim * drho
+    assert F[i, 5] == pytest.approx(want, rel=1e-12)
+    r_other = (1.9 - 1.3) / drho
+    lim_other = (r_other + abs(r_other)) / (1.0 + abs(r_other))
+    other = v_tc * 1.3 + 0.5 * abs(v_tc) * (1.0 - abs(v_tc * dt_h / h)) * lim_other * drho
+    assert abs(other - want) > 1e-3 * abs(want)
+
+
 # ------------------------------------------------------ whole-step invariants ---
 
 def _run(po, n, steps, init):
