@@ -5,6 +5,9 @@
  * ...", "flow.params density=... energy=..."). '#' starts a comment line.
  */
 #include <ctype.h>
+#include <errno.h>
+#include <limits.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -12,14 +15,23 @@
 #include "../../../include/flow_arch.h"
 
 /* Finds the first line whose first token equals `key`; copies the rest of
- * the line (after the token) into rest. Returns 1 if found. */
+ * the line (after the token) into rest. Returns 1 if found. A line longer
+ * than the reader's buffer is an error (read in pieces, its tail would pass
+ * for lines of its own). */
 static int find_line(const char* key, const char* filename, char* rest,
                      size_t rest_len) {
   FILE* f = fopen(filename, "r");
   if (!f) return -1;
   char line[MAX_STR_LEN * 4];
   int found = 0;
+  int lineno = 0;
   while (fgets(line, sizeof(line), f)) {
+    ++lineno;
+    if (!strchr(line, '\n') && !feof(f)) {
+      fclose(f);
+      TERMINATE("Line %d of %s is longer than %d characters.", lineno, filename,
+                (int)sizeof(line) - 2);
+    }
     char* p = line;
     while (*p && isspace((unsigned char)*p)) ++p;
     if (*p == '\0' || *p == '#') continue;
@@ -58,9 +70,13 @@ int get_int_parameter(const char* param_name, const char* filename) {
   char tok[MAX_STR_LEN];
   value_token(param_name, filename, tok, sizeof(tok));
   char* end = NULL;
+  errno = 0;
   const long v = strtol(tok, &end, 10);
   if (*end != '\0')
     TERMINATE("The parameter %s in file %s is not an integer: %s", param_name,
+              filename, tok);
+  if (errno == ERANGE || v < INT_MIN || v > INT_MAX)
+    TERMINATE("The parameter %s in file %s is out of range: %s", param_name,
               filename, tok);
   return (int)v;
 }
@@ -69,9 +85,13 @@ double get_double_parameter(const char* param_name, const char* filename) {
   char tok[MAX_STR_LEN];
   value_token(param_name, filename, tok, sizeof(tok));
   char* end = NULL;
+  errno = 0;
   const double v = strtod(tok, &end);
   if (*end != '\0')
     TERMINATE("The parameter %s in file %s is not a number: %s", param_name,
+              filename, tok);
+  if (errno == ERANGE && isinf(v))
+    TERMINATE("The parameter %s in file %s is out of range: %s", param_name,
               filename, tok);
   return v;
 }
