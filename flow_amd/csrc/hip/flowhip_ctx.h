@@ -115,15 +115,14 @@ void flowhip_invalidate_sums(Mesh* mesh);
 FkReduce flowhip_lagrangian_red(FlowHipCtx* h, int nx, int ny);
 
 /* Chunk height of one march kernel. An explicit height (flowhip_set_march_rows,
- * FLOWHIP_MARCH_ROWS) applies to all of them; by default each kernel uses
- * the height measured fastest for it on MI355X at 8192^2. Short chunks cost
- * their prologue rows again; with items claimed from a queue the waves end
- * within an item of each other, so shorter items shorten the tail. Every
- * height gives bitwise the same result. */
+ * FLOWHIP_MARCH_ROWS) gives uniform chunks of that height to all of them;
+ * 0, the default, lets each kernel use the chunking measured fastest for it
+ * on MI355X (march.hip). Every chunking gives bitwise the same result. */
 enum { FLOWHIP_K_LAGRANGIAN = 0, FLOWHIP_K_MASS = 1, FLOWHIP_K_MOMENTUM = 2 };
 static inline int flowhip_rows(const FlowHipCtx* h, int kernel, int fp32) {
-  static const int kRows[2][3] = {{24, 20, 16}, {28, 16, 20}};
-  return h->march_rows > 0 ? h->march_rows : kRows[fp32 ? 1 : 0][kernel];
+  (void)kernel;
+  (void)fp32;
+  return h->march_rows;
 }
 
 #endif

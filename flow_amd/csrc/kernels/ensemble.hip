@@ -58,6 +58,13 @@ __device__ __forceinline__ MarchList ens_list(const FkEnsInst& in, int H) {
   ml.nstrips = in.nstrips;
   ml.nchunks = in.nchunks;
   ml.nitems = in.nitems;
+  /* uniform chunks of H rows */
+  ml.G = 1;
+  ml.cpb = in.nchunks;
+  ml.H2 = H;
+  ml.k = 1;
+  ml.nsmall = 0;
+  ml.R = in.ny - 2 * kPad;
   return ml;
 }
 

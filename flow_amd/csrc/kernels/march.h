@@ -64,10 +64,49 @@ constexpr int kMWaves = kMNT / kWave;
 template <typename T>
 constexpr int kMarchMaxWaves = sizeof(T) == 8 ? 3 : 5;
 
+/* The work items of a march kernel: strips of W columns by chunks of rows.
+ * The interior rows are cut into G row bands (one per XCD band of the
+ * item list, MarchQueue), each into units of H2 rows. A band's chunks are
+ * k units tall (H = k * H2 rows) except its last `nsmall`, one unit each:
+ * the waves claim tall chunks first, which re-read few prologue rows per
+ * row marched, and finish on short ones, which keeps the band's tail short.
+ * Every band has cpb chunk slots (a shorter band leaves its last slot
+ * empty); items run strip-fastest within a chunk, band by band. Every chunk
+ * but the mesh's first starts at least H2 rows above the south wall. With
+ * G = 1, k = 1 and nsmall = 0 the chunks are uniform, H2 rows each. */
 struct MarchList {
-  int W, H;           /* strip width (output columns), chunk height (rows) */
+  int W, H;           /* strip width (output columns), tall chunk height (rows) */
   int nstrips, nchunks, nitems;
+  int G, cpb;         /* row bands, chunk slots per band */
+  int H2, k, nsmall;  /* unit rows, units per tall chunk, short chunks per band */
+  int R;              /* interior rows */
 };
+
+/* Columns and rows of one item; y0 == y1 for an empty slot. */
+struct ItemBox {
+  int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ ItemBox item_box(const MarchList& ml, int nx, int item) {
+  const int strip = item % ml.nstrips, slot = item / ml.nstrips;
+  const int g = slot / ml.cpb, c = slot - g * ml.cpb;
+  const int r0 = kPad + g * ml.R / ml.G, r1 = kPad + (g + 1) * ml.R / ml.G;
+  const int units = (r1 - r0 + ml.H2 - 1) / ml.H2;
+  const int ub = max(units - ml.nsmall, 0);           /* units in tall chunks */
+  const int nbig = (ub + ml.k - 1) / ml.k;
+  ItemBox b;
+  b.x0 = kPad + strip * ml.W;
+  b.x1 = min(b.x0 + ml.W, nx - kPad);
+  if (c < nbig) {
+    b.y0 = r0 + c * ml.k * ml.H2;
+    b.y1 = min(r0 + min((c + 1) * ml.k, ub) * ml.H2, r1);
+  } else {
+    const int u = ub + (c - nbig);
+    b.y0 = min(r0 + u * ml.H2, r1);
+    b.y1 = u < units ? min(b.y0 + ml.H2, r1) : b.y0;
+  }
+  return b;
+}
 
 /* A wave's work items under the static banded walk: first, first + stride,
  * ... below end. The item list is cut into B = min(kMarchBands, gridDim)
@@ -179,12 +218,26 @@ struct MarchBlockQueue : MarchQueue<true> {
   }
 };
 
-inline MarchList make_march(int nx, int ny, int W, int H) {
+/* Items of a mesh: H2-row units in tall chunks of k units, the last nsmall
+ * chunks of each of G row bands one unit each (G is lowered until every
+ * band holds at least 4 units; k = 1, nsmall = 0 gives uniform chunks). */
+inline MarchList make_march(int nx, int ny, int W, int H2, int k = 1, int nsmall = 0,
+                            int G = 1) {
   MarchList m;
   m.W = W;
-  m.H = H;
+  m.H2 = H2;
+  m.k = k;
+  m.nsmall = nsmall;
+  m.H = k * H2;
+  m.R = ny - 2 * kPad;
+  m.G = G;
+  while (m.G > 1 && m.R / m.G < 4 * H2) --m.G;
+  const int rmax = (m.R + m.G - 1) / m.G;
+  const int units = (rmax + H2 - 1) / H2;
+  const int ub = units > nsmall ? units - nsmall : 0;
+  m.cpb = (ub + k - 1) / k + (units < nsmall ? units : nsmall);
   m.nstrips = (nx - 2 * kPad + W - 1) / W;
-  m.nchunks = (ny - 2 * kPad + H - 1) / H;
+  m.nchunks = m.G * m.cpb;
   m.nitems = m.nstrips * m.nchunks;
   return m;
 }
@@ -231,6 +284,15 @@ __device__ __forceinline__ T dn(T v, int d) {
 __device__ __forceinline__ bool item_xedge(const MarchList& ml, int nx, int item) {
   const int x0 = kPad + (item % ml.nstrips) * ml.W, x1 = min(x0 + ml.W, nx - kPad);
   return (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
+}
+
+/* Whether an item's stencils reach a ghost row or column anywhere (the
+ * x test above, or a chunk within three rows of a north/south wall); the
+ * other items, all but the mesh's outer ring of items, run the
+ * straight-line interior form of their kernel. */
+__device__ __forceinline__ bool item_edge(const MarchList& ml, int nx, int ny, int item) {
+  const ItemBox b = item_box(ml, nx, item);
+  return item_xedge(ml, nx, item) || (b.y0 == kPad) || (b.y1 + 3 > ny - kPad);
 }
 
 /* ---- per-row geometry of a chunk held across the lanes ----
@@ -360,16 +422,16 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * [x0, x1] and the first (x) update of cells [x0, x1); per lane a window of
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
-template <typename T, bool P2>
+template <typename T, bool P2, bool EDGE = true>
 __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                    (y1 + 3 > ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                             (y1 + 3 > ny - kPad));
   const int q = x0 - 2 + lane;
   const bool inq = q < nx;                     /* column exists */
   const bool face = q >= x0 && q <= x1;        /* lane computes face q */
@@ -393,7 +455,7 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   T vy[3];   /* v0 at faces r-2 .. r */
   T r0[3];   /* rho0 at rows r-2 .. r */
   T fy_prev = T(0), efy_prev = T(0);
-  const bool south = (y0 == kPad);
+  const bool south = EDGE && (y0 == kPad);
   const int rb = south ? kPad : y0 - 2;
 #pragma unroll
   for (int k = 0; k < 4; ++k) wr[k] = we[k] = T(0);
@@ -405,7 +467,7 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   T a_r = io.rho0[rb * nx + qc], a_e = io.e1[rb * nx + qc];
   T a_u = io.u0[rb * sx + qf], a_v = io.v0[rb * nx + qc];
   for (int r = rb; r <= y1 + 1; ++r) {
-    const bool ghost_row = r >= ny - kPad;
+    const bool ghost_row = EDGE && r >= ny - kPad;
     /* prefetch the next row */
     const int rn = min(r + 1, ny - kPad - 1);
     const T n_r = io.rho0[rn * nx + qc], n_e = io.e1[rn * nx + qc];
@@ -500,7 +562,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
-    mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    if (item_edge(ml, g.nx, g.ny, item))
+      mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    else
+      mass_xy_item<T, P2, false>(g, io, dt, dt_h, ml, item, lane, ms, es);
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -516,23 +581,23 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * column (the same column inside the domain). Per row r: the y flux at face
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
-template <typename T, bool P2>
+template <typename T, bool P2, bool EDGE = true>
 __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                    (y1 + 3 > ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                             (y1 + 3 > ny - kPad));
   const int q = x0 - 2 + lane;
   const bool inq = q < nx;
   const bool face = q >= x0 && q <= x1;
   const bool own = q >= x0 && q < x1;
   const bool own_face = face && (q < x1 || x1 == nx - kPad);
   /* the y sweep of a ghost column is that of its mirror column */
-  const int qm = inq ? mirror(q, nx) : nx - kPad - 1;
+  const int qm = !EDGE ? q : (inq ? mirror(q, nx) : nx - kPad - 1);
   const int qc = inq ? q : nx - 1;
   const int qf = q <= nx ? q : nx;
   const T cdx_m = g.cdx[qm], edx_m = g.edx[qm], edx_m1 = g.edx[qm + 1];
@@ -642,7 +707,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
-    mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    if (item_edge(ml, g.nx, g.ny, item))
+      mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    else
+      mass_yx_item<T, P2, false>(g, io, dt, dt_h, ml, item, lane, ms, es);
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -699,9 +767,9 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
   const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
@@ -808,9 +876,9 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
   const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
@@ -905,9 +973,9 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
   const int gend = (y1 == ny - kPad) ? y1 + 1 : y1;
@@ -1048,9 +1116,9 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
   const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
                     (y1 + 3 > ny - kPad);
   const bool xedge = (x0 - 3 < kPad) || (x1 + 3 > nx - kPad);
@@ -1173,21 +1241,25 @@ struct MLagIO {
  * column carries v after pressure acceleration on y faces i .. i+3 (face
  * i+3 computed this step), Qyy on cell rows i and i+1 and v after the
  * viscous acceleration on faces i and i+1. Ghost faces hold the negated
- * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell. */
-template <typename T, bool P2, int XE = -1>
+ * mirror (INVERT_X / INVERT_Y), ghost Q cells the mirror cell.
+ * EDGE = false is the interior form, for items that item_edge clears: no
+ * ghost store, mirror or wall test is compiled, so each row is one
+ * straight-line block. */
+template <typename T, bool P2, int XE = -1, bool EDGE = true>
 __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, double& local_min, int& nan_count) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                    (y1 + 3 > ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                             (y1 + 3 > ny - kPad));
   /* strips within three columns of a wall hold ghost lanes; XE fixes that at
    * compile time (0: none, 1: some), -1 tests it here */
-  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
-  const bool last_chunk = (y1 == ny - kPad);
+  const bool xedge =
+      EDGE && (XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad)));
+  const bool last_chunk = EDGE && (y1 == ny - kPad);
   const int q = x0 - 3 + lane;
   const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
   const int qg = min(max(q, 1), nx);
@@ -1231,7 +1303,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     for (int s = 0; s < 5; ++s) {
       const int f = y0 - 2 + s;
       pm[s] = pr[s] = T(0);
-      if (f >= kPad && f < ny - 1)
+      if (!EDGE || (f >= kPad && f < ny - 1))
         vp[s] = ypa(f, rr[s + 1], ee[s + 1], rr[s], ee[s], io.mom_y[f * nx + qc],
                     pm[s], pr[s]);
       else if (f == ny - 1)
@@ -1241,7 +1313,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
       else
         vp[s] = T(0);
     }
-    if (y0 == kPad) {
+    if (EDGE && y0 == kPad) {
       vp[0] = -vp[3];
       vp[1] = -vp[2];
     }
@@ -1252,8 +1324,8 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     }
     /* Qyy at cell rows y0 and y0-1 (below the south wall: row y0's) */
     qy_prev = viscous_stress(rr[3], vp[1], vp[2], vp[3], vp[4]);
-    qy_m1 = (y0 == kPad) ? qy_prev
-                         : viscous_stress(rr[2], vp[0], vp[1], vp[2], vp[3]);
+    qy_m1 = (EDGE && y0 == kPad) ? qy_prev
+                                 : viscous_stress(rr[2], vp[0], vp[1], vp[2], vp[3]);
   }
   T vav_prev;
   {
@@ -1293,10 +1365,11 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
       a_r = io.rho[r5 * nx + qc];
       a_e = io.e[r5 * nx + qc];
     }
-    /* y face i+3 after pressure acceleration */
+    /* y face i+3 after pressure acceleration (an interior item's faces all
+     * lie below the north ghost faces) */
     const int k = i + 3;
     T vk;
-    if (k < ny - 1) {
+    if (!EDGE || k < ny - 1) {
       vk = ypa(k, wr[3], we[3], wr[2], we[2], c_my, wm[2], wx[2]);
     } else {
       vk = -((k == ny - 1) ? wv[2] : wv[0]);
@@ -1304,8 +1377,9 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
     }
     wv[3] = vk;
     /* Qyy at cell row i+1 (a ghost row is its mirror, row i) */
-    const T qy = (i + 1 >= ny - kPad) ? qy_prev
-                                       : viscous_stress(wr[1], wv[0], wv[1], wv[2], wv[3]);
+    const T qy = (EDGE && i + 1 >= ny - kPad)
+                     ? qy_prev
+                     : viscous_stress(wr[1], wv[0], wv[1], wv[2], wv[3]);
     /* y face i+1 after the viscous acceleration */
     const T my_av = accelerate<T, P2>(wm[0], dt, qy, qy_prev, cdy(i + 1), rcdy(i + 1));
     const T vav = velocity(my_av, wx[0]);
@@ -1399,7 +1473,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   int nan_count = 0;
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
-    if (item_xedge(ml, g.nx, item))
+    if (!item_edge(ml, g.nx, g.ny, item))
+      lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane, local_min,
+                                       nan_count);
+    else if (item_xedge(ml, g.nx, item))
       lagrangian_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
     else
       lagrangian_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
@@ -1436,21 +1513,24 @@ struct MMom2IO {
  * columns: the y stage's corner flux at face x1 needs u at x1, hence the x
  * stage's cell flux at x1 and u0 up to x1+2). The per-row arithmetic is the
  * two separate kernels', so the results are theirs bit for bit. */
-template <typename T, bool P2, int XE = -1>
+template <typename T, bool P2, int XE = -1, bool EDGE = true>
 __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                    (y1 + 3 > ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                             (y1 + 3 > ny - kPad));
   /* strips within three columns of a wall hold ghost lanes; XE fixes that at
    * compile time (0: none, 1: some), -1 tests it here */
-  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
-  const bool last_chunk = (y1 == ny - kPad);
+  const bool xedge =
+      EDGE && (XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad)));
+  const bool last_chunk = EDGE && (y1 == ny - kPad);
+  /* the row a ghost row mirrors (an interior item has none) */
+  auto ymir = [&](int r) { return EDGE ? mirror(r, ny) : r; };
   const int gend = last_chunk ? y1 + 1 : y1;
   const int q = x0 - 2 + lane;
   const int qf = min(q, nx), qc = min(q, nx - 1), qg = min(max(q, 1), nx);
@@ -1511,7 +1591,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   T uu[5], ff[5], rr[5], wm[3];
 #pragma unroll
   for (int s = 0; s < 5; ++s) {
-    const int i = mirror(y0 - 3 + s, ny);
+    const int i = ymir(y0 - 3 + s);
     const T a_fx = io.F_x[i * sx + qf], a_r = io.rho[i * nx + qc];
     T m1, u1;
     xrow(i, io.u0[i * sx + qf], a_fx, io.mx1[i * sx + qf], a_r, m1, u1);
@@ -1533,14 +1613,14 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   for (int s = 0; s < 4; ++s) {
     const int k = y0 - 2 + s;
     wv[s] = T(0);
-    if (k < kPad) continue;
+    if (EDGE && k < kPad) continue;
     T m1, v1;
     yrow(k, io.v0[k * nx + qc], io.my1[k * nx + qc], ff[s + 1], ff[s], uu[s + 1],
          uu[s], rr[s + 1], rr[s], m1, v1);
     wv[s] = v1;
     if (s >= 2) wn[s - 2] = m1;
   }
-  const bool south = (y0 == kPad);
+  const bool south = EDGE && (y0 == kPad);
   if (south) {
     wv[0] = -wv[3];
     wv[1] = -wv[2];
@@ -1554,7 +1634,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   T p_fx = ff[4], p_u = uu[4], p_r = rr[4];
   T p_v = io.v0[(y0 + 1) * nx + qc];
   /* inputs of the lookahead row y0+2 */
-  int ia = mirror(y0 + 2, ny);
+  int ia = ymir(y0 + 2);
   T a_u = io.u0[ia * sx + qf], a_fx = io.F_x[ia * sx + qf];
   T a_mx = io.mx1[ia * sx + qf], a_r = io.rho[ia * nx + qc];
   T a_v = io.v0[min(y0 + 2, ny) * nx + qc], a_my = io.my1[min(y0 + 2, ny) * nx + qc];
@@ -1563,7 +1643,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
     const T c_u = a_u, c_fx = a_fx, c_mx = a_mx, c_r = a_r, c_v = a_v, c_my = a_my,
             c_fy = a_fy;
     {
-      const int in = mirror(r + 3, ny), ry = min(r + 3, ny);
+      const int in = ymir(r + 3), ry = min(r + 3, ny);
       a_u = io.u0[in * sx + qf];
       a_fx = io.F_x[in * sx + qf];
       a_mx = io.mx1[in * sx + qf];
@@ -1575,12 +1655,12 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
     const int k = r + 2;
     /* x stage on lookahead row k: u there feeds both stages */
     T m1x, u1;
-    xrow(mirror(k, ny), c_u, c_fx, c_mx, c_r, m1x, u1);
+    xrow(ymir(k), c_u, c_fx, c_mx, c_r, m1x, u1);
     wu[3] = u1;
     wm[2] = m1x;
     /* y stage on lookahead face k */
     T m1y = T(0), v1;
-    if (k < ny - 1)
+    if (!EDGE || k < ny - 1)
       yrow(k, c_v, c_my, c_fx, p_fx, u1, p_u, c_r, p_r, m1y, v1);
     else
       /* ghost y face: mirror of face ny-1 is ny-2 (k-1), of ny is ny-3 (k-3) */
@@ -1598,7 +1678,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
       uf_prev = uf;
     }
     /* y stage output: y-face row r */
-    const T vfy = (r >= ny - kPad)
+    const T vfy = (EDGE && r >= ny - kPad)
                       ? vfy_prev
                       : cellflux(r, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
     if (south && r == y0) vfy_prev = vfy;
@@ -1629,21 +1709,22 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
  * flux in x) and y momentum (y flux in y, v recompute, y flux in x) in one
  * march; the y stage at row r reads u of rows r and r-1 from the x stage.
  * Lane l holds x face / column q = x0 - 3 + l (strips of 59 columns). */
-template <typename T, bool P2, int XE = -1>
+template <typename T, bool P2, int XE = -1, bool EDGE = true>
 __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
-  const int strip = item % ml.nstrips, chunk = item / ml.nstrips;
-  const int x0 = kPad + strip * ml.W, x1 = min(x0 + ml.W, nx - kPad);
-  const int y0 = kPad + chunk * ml.H, y1 = min(y0 + ml.H, ny - kPad);
-  const bool edge = (x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
-                    (y1 + 3 > ny - kPad);
+  const ItemBox box = item_box(ml, nx, item);
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
+                             (y1 + 3 > ny - kPad));
   /* strips within three columns of a wall hold ghost lanes; XE fixes that at
    * compile time (0: none, 1: some), -1 tests it here */
-  const bool xedge = XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad));
-  const bool last_chunk = (y1 == ny - kPad);
+  const bool xedge =
+      EDGE && (XE >= 0 ? XE != 0 : ((x0 - 3 < kPad) || (x1 + 3 > nx - kPad)));
+  const bool last_chunk = EDGE && (y1 == ny - kPad);
   const int gend = last_chunk ? y1 + 1 : y1;
   const int q = x0 - 3 + lane;
   const int qf = min(max(q, 0), nx), qc = min(max(q, 0), nx - 1);
@@ -1690,7 +1771,7 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   }
   const T fy0 = io.F_y[y0 * nx + qc];
   T uf_prev = corner(y0, wu[0], wu[1], wu[2], wu[3], wv[2], fy0);
-  const bool south = (y0 == kPad);
+  const bool south = EDGE && (y0 == kPad);
   T vfy_prev = south ? T(0)
                      : cellflux(y0 - 1, wv[0], wv[1], wv[2], wv[3],
                                 io.F_y[(y0 - 1) * nx + qc], fy0);
@@ -1751,7 +1832,7 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
     }
     if (south && r == y0) p_u = u1;
     /* y stage: y-face row r */
-    const T vfy = (r >= ny - kPad)
+    const T vfy = (EDGE && r >= ny - kPad)
                       ? vfy_prev
                       : cellflux(r, wv[0], wv[1], wv[2], wv[3], p_fy, c_fy);
     if (south && r == y0) vfy_prev = vfy;
@@ -1793,7 +1874,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
   for (int base = mq.next(); base < mq.end; base = mq.next()) {
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
-    if (item_xedge(ml, g.nx, item))
+    if (!item_edge(ml, g.nx, g.ny, item))
+      momxy_xy_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane);
+    else if (item_xedge(ml, g.nx, item))
       momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
       momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
@@ -1811,7 +1894,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
   for (int base = mq.next(); base < mq.end; base = mq.next()) {
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
-    if (item_xedge(ml, g.nx, item))
+    if (!item_edge(ml, g.nx, g.ny, item))
+      momxy_yx_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane);
+    else if (item_xedge(ml, g.nx, item))
       momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
     else
       momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);

@@ -6,13 +6,60 @@
 
 namespace flowhip {
 
+/* Chunking of a march kernel when no chunk height is given (chunk_rows 0):
+ * units of H2 rows, chunks of k units, and nsmall one-unit chunks at the end
+ * of each of the 8 XCD row bands (MarchList). Measured on MI355X at 8192^2,
+ * uniform chunks (k = 1, nsmall = 0) of 24 / 20 / 16 rows (Lagrangian /
+ * mass-energy / momentum; 24 / 16 / 20 in FP32) are fastest: tapering from
+ * tall chunks (48-80 rows) down to 12-24-row ones cut the re-read prologue
+ * rows but made the Lagrangian and mass/energy marches 5-10% slower. An
+ * explicit chunk_rows > 0 gives uniform chunks of that height. Every
+ * chunking gives bitwise the same result. */
+struct Taper {
+  int H2, k, nsmall;
+};
+#ifndef FLOWHIP_TAPER_LAG
+#define FLOWHIP_TAPER_LAG 24, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_MASS
+#define FLOWHIP_TAPER_MASS 20, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_MOM
+#define FLOWHIP_TAPER_MOM 16, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_LAG32
+#define FLOWHIP_TAPER_LAG32 24, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_MASS32
+#define FLOWHIP_TAPER_MASS32 16, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_MOM32
+#define FLOWHIP_TAPER_MOM32 20, 1, 0
+#endif
+enum { kKLag = 0, kKMass = 1, kKMom = 2 };
+template <typename T>
+constexpr Taper taper_of(int kernel) {
+  return sizeof(T) == 8 ? (kernel == kKLag    ? Taper{FLOWHIP_TAPER_LAG}
+                           : kernel == kKMass ? Taper{FLOWHIP_TAPER_MASS}
+                                              : Taper{FLOWHIP_TAPER_MOM})
+                        : (kernel == kKLag    ? Taper{FLOWHIP_TAPER_LAG32}
+                           : kernel == kKMass ? Taper{FLOWHIP_TAPER_MASS32}
+                                              : Taper{FLOWHIP_TAPER_MOM32});
+}
+
+inline MarchList march_items(int nx, int ny, int W, int chunk_rows, Taper t) {
+  if (chunk_rows > 0)
+    return make_march(nx, ny, W, chunk_rows < kMaxChunkRows ? chunk_rows : kMaxChunkRows);
+  const int k = t.k * t.H2 <= kMaxChunkRows ? t.k : kMaxChunkRows / t.H2;
+  return make_march(nx, ny, W, t.H2, k, t.nsmall, kMarchBands);
+}
+
 /* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
 constexpr int kMassW = kWave - 4;
 template <typename T, bool P2>
 hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
-  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
-  const MarchList ml = make_march(g.nx, g.ny, kMassW, chunk_rows);
+  const MarchList ml = march_items(g.nx, g.ny, kMassW, chunk_rows, taper_of<T>(kKMass));
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
@@ -31,8 +78,7 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
 template <typename T, bool P2>
 hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                      int first_step, int chunk_rows, hipStream_t s) {
-  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
-  const MarchList ml = make_march(g.nx, g.ny, kWave - 6, chunk_rows);
+  const MarchList ml = march_items(g.nx, g.ny, kWave - 6, chunk_rows, taper_of<T>(kKLag));
   if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
   const int grid = march_grid(k_march_lagrangian<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
@@ -48,10 +94,10 @@ constexpr int kMomW = kWave - 4;
 template <typename T, bool P2>
 hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
                                    const double* dtp, int chunk_rows, hipStream_t s) {
-  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
   const bool odd = (tt % 2) != 0;
   const MarchList ml =
-      make_march(g.nx, g.ny, (!ydir && !odd) ? kMomXW : kMomW, chunk_rows);
+      march_items(g.nx, g.ny, (!ydir && !odd) ? kMomXW : kMomW, chunk_rows,
+                  taper_of<T>(kKMom));
   if (ml.nitems < 1) return hipErrorInvalidValue;
   if (!ydir && odd) {
     static int per_cu = 0;
@@ -79,8 +125,7 @@ constexpr int kMomXYW = kWave - 5;
 template <typename T, bool P2>
 hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
                                 unsigned int* queue, int chunk_rows, hipStream_t s) {
-  if (chunk_rows > kMaxChunkRows) chunk_rows = kMaxChunkRows;
-  const MarchList ml = make_march(g.nx, g.ny, kMomXYW, chunk_rows);
+  const MarchList ml = march_items(g.nx, g.ny, kMomXYW, chunk_rows, taper_of<T>(kKMom));
   if (ml.nitems < 1 || !queue) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
