@@ -229,6 +229,12 @@ int main(int argc, char** argv) {
                                                                    : "defined-minimum");
 
   const int result = validate_run(&mesh, opt.params, sd.density, sd.energy);
+  /* main.c:123-124 with profiling on: the kernel times of this run */
+  const char* prof = getenv("FLOWHIP_PROFILE");
+  if (prof && !strcmp(prof, "1")) {
+    flowhip_print_profile(&compute_profile);
+    flowhip_print_profile(&comms_profile);
+  }
   printf("Wallclock %.4fs, Elapsed Simulation Time %.4fs\n", wallclock, sim_time);
   if (visit_dump)
     write_all_ranks_to_visit(mesh.global_nx + 2 * mesh.pad, mesh.global_ny + 2 * mesh.pad,

@@ -3,6 +3,7 @@
  * per-kernel event timers.
  */
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include "flowhip_ctx.h"
@@ -39,6 +40,23 @@ static void schedule_from_env(FlowHipCtx* h) {
       TERMINATE("FLOWHIP_MARCH_ROWS must be an integer in [4, 120], not '%s'", r);
     h->march_rows = (int)rows;
   }
+}
+
+/* Live contexts, for flowhip_print_profile. */
+#define FLOWHIP_MAX_LIVE 64
+static FlowHipCtx* g_live[FLOWHIP_MAX_LIVE];
+
+static void live_add(FlowHipCtx* h) {
+  for (int k = 0; k < FLOWHIP_MAX_LIVE; ++k)
+    if (!g_live[k]) {
+      g_live[k] = h;
+      return;
+    }
+}
+
+static void live_remove(FlowHipCtx* h) {
+  for (int k = 0; k < FLOWHIP_MAX_LIVE; ++k)
+    if (g_live[k] == h) g_live[k] = NULL;
 }
 
 FlowHipCtx* flowhip_ctx(Mesh* mesh) {
@@ -80,12 +98,16 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
   h->no_pow2 = (p2 && !strcmp(p2, "0")) ? 1 : 0;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
+  live_add(h);
+  const char* prof = getenv("FLOWHIP_PROFILE");
+  if (prof && !strcmp(prof, "1")) flowhip_set_timing(mesh, 1);
   return h;
 }
 
 void flowhip_ctx_destroy(Mesh* mesh) {
   FlowHipCtx* h = (FlowHipCtx*)mesh->hip;
   if (!h) return;
+  live_remove(h);
   hipStreamSynchronize(h->stream);
   for (int g = 0; g < 3; ++g)
     if (h->graph_ok[g]) hipGraphExecDestroy(h->graph[g]);
@@ -336,4 +358,47 @@ FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
   f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
   f.pow2 = h->pow2_f32 && !h->no_pow2;
   return f;
+}
+
+/* arch's PRINT_PROFILING_RESULTS: the hipEvent times of every live mesh's
+ * kernels (meshes created with FLOWHIP_PROFILE=1), longest first. */
+void flowhip_print_profile(const struct Profile* p) {
+  if (p == &comms_profile) {
+    printf("Communication: none (one GPU; the reflective boundary and the "
+           "reductions run on the device)\n");
+    return;
+  }
+  int any = 0;
+  for (int m = 0; m < FLOWHIP_MAX_LIVE; ++m) {
+    FlowHipCtx* h = g_live[m];
+    if (!h || !h->ntimers) continue;
+    if (h->npending) {
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      flowhip_timer_collect(h);
+    }
+    int order[FLOWHIP_MAX_TIMERS];
+    for (int k = 0; k < h->ntimers; ++k) order[k] = k;
+    for (int a = 1; a < h->ntimers; ++a)
+      for (int b = a; b > 0 && h->timers[order[b]].total_ms >
+                                   h->timers[order[b - 1]].total_ms; --b) {
+        const int t = order[b];
+        order[b] = order[b - 1];
+        order[b - 1] = t;
+      }
+    double total = 0.0;
+    for (int k = 0; k < h->ntimers; ++k) total += h->timers[k].total_ms;
+    printf("\nKernel times (hipEvents on the mesh's stream):\n");
+    printf("%-24s %10s %14s %14s %8s\n", "kernel", "launches", "total (ms)",
+           "per launch (ms)", "share");
+    for (int k = 0; k < h->ntimers; ++k) {
+      const FlowHipTimer* t = &h->timers[order[k]];
+      printf("%-24s %10ld %14.3f %14.4f %7.1f%%\n", t->name, t->launches, t->total_ms,
+             t->launches ? t->total_ms / (double)t->launches : 0.0,
+             total > 0.0 ? 100.0 * t->total_ms / total : 0.0);
+    }
+    any = 1;
+  }
+  if (!any)
+    printf("Kernel times: none recorded (set FLOWHIP_PROFILE=1 to time the "
+           "kernels)\n");
 }

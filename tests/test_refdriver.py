@@ -14,6 +14,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 REFDIR = os.path.join(HERE, "refdriver")
 BINARY = os.path.join(REFDIR, "_ref", "flow_ref_driver")
+BINARY_PROF = os.path.join(REFDIR, "_ref", "flow_ref_driver_prof")
 REFERENCE = "/root/reference"
 LIB = os.path.join(os.path.dirname(HERE), "flow_amd", "lib", "libflowhip.so")
 
@@ -73,3 +74,30 @@ def test_reference_driver_known_answer(tmp_path, schedule):
     assert f"Expected energy  {kat['energy']:.12e}, result was {kat['energy']:.12e}." in out
     assert "PASSED validation." in out
     assert out.count("\nIteration ") == kat["iterations"]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(BINARY_PROF),
+                    reason="flow's driver was not linked (needs the reference checkout)")
+def test_reference_driver_prints_kernel_times(tmp_path):
+    """main.c built with -DENABLE_PROFILING (arch's switch) and run with
+    FLOWHIP_PROFILE=1: PRINT_PROFILING_RESULTS (main.c:123-124) lists the
+    kernel set's hipEvent times, one launch of each march kernel per
+    iteration, and the known answer still passes."""
+    run, kat = _write_case(tmp_path)
+    env = dict(os.environ, FLOWHIP_PROFILE="1")
+    print(f"case flow driver profiling n={kat['nx']}", flush=True)
+    r = subprocess.run([BINARY_PROF, "flow.params"], cwd=run, env=env,
+                       capture_output=True, text=True, timeout=110)
+    out = r.stdout
+    assert r.returncode == 0, out[-2000:] + r.stderr[-2000:]
+    assert "PASSED validation." in out
+    assert "Kernel times (hipEvents" in out
+    rows = {line.split()[0]: line.split() for line in out.splitlines()
+            if line.split() and line.split()[0] in ("lagrangian", "advect_mass_energy",
+                                                   "advect_momentum", "timestep")}
+    for k in ("lagrangian", "advect_mass_energy", "advect_momentum"):
+        assert int(rows[k][1]) == kat["iterations"], rows[k]
+        assert float(rows[k][2]) > 0.0
+    assert int(rows["timestep"][1]) == 1   # the pre-loop set_timestep
+    assert "Communication: none" in out
