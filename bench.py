@@ -18,6 +18,7 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -31,6 +32,10 @@ METRIC = ("cell-updates/s and % HBM peak at 8192^2, FP64 vs FP32 state; "
 HBM_PEAK = 8.0e12            # MI355X HBM3E, B/s
 B_ALG_FP64 = 376             # bytes per cell-update: 47 compulsory field sweeps
 B_ALG_FP32 = 188
+# The march schedule's own compulsory traffic: its three kernels read and
+# write 10 + 8 + 11 = 29 field sweeps per cell-update.
+B_MARCH_FP64 = 232
+B_MARCH_FP32 = 116
 
 # Algorithmic bytes per launch, in field sweeps of 8 B over the interior
 # (each distinct array a kernel reads counted once, each it writes once).
@@ -65,6 +70,8 @@ def parse_args():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="approximate CPU time budget of the oracle sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-other-configs", action="store_true",
+                   help="skip the Sod 512^2 and 4096^2 blast lines")
     p.add_argument("--no-fp32", action="store_true",
                    help="skip the FP32-state precision form")
     p.add_argument("--fp32-rows", type=int, default=None,
@@ -101,6 +108,25 @@ def traffic_from_profiles(kernel, n):
         if m and m.group(1) == kernel and int(m.group(2)) == n:
             return float(m.group(3))
     return None
+
+
+def step_roofline(cups, s_per_step, n, sched_name, prof_run, b_alg, b_sched, suffix=""):
+    """The whole step against HBM peak three ways: at the reference
+    schedule's fixed B_ALG (kept for comparability), at this schedule's own
+    compulsory bytes, and at the bytes rocprofv3's counters measured per
+    step (the three march kernels' committed traffic lines), when known."""
+    out = {"b_alg_per_cell_update": b_alg,
+           "achieved_GBps": cups * b_alg / 1e9,
+           "frac": cups * b_alg / HBM_PEAK}
+    if sched_name == "march":
+        out["schedule_b_per_cell_update"] = b_sched
+        out["schedule_frac"] = cups * b_sched / HBM_PEAK
+        parts = [traffic_from_profiles(f"{prof_run}:{k}{suffix}", n)
+                 for k in ("lagrangian", "advect_mass_energy", "advect_momentum")]
+        if all(p is not None for p in parts):
+            out["pmc_bytes_per_step"] = sum(parts)
+            out["pmc_frac"] = sum(parts) / s_per_step / HBM_PEAK
+    return out
 
 
 def field_errors(got, ref, L):
@@ -164,9 +190,8 @@ def fp32_leg(args, n, rho0, e0, ref_state, ref_dts, ref_mass, ref_energy,
     return {
         "value": cups, "unit": "cell-updates/s", "ms_per_step": elapsed / steps * 1e3,
         "speedup_vs_fp64": cups / fp64_cups,
-        "step_roofline": {"b_alg_per_cell_update": B_ALG_FP32,
-                          "achieved_GBps": cups * B_ALG_FP32 / 1e9,
-                          "frac": cups * B_ALG_FP32 / HBM_PEAK},
+        "step_roofline": step_roofline(cups, elapsed / steps, n, "march", "fp32",
+                                       B_ALG_FP32, B_MARCH_FP32, suffix="_f32"),
         "error_vs_fp64": errs,
         "max_field_rel_linf": max(v["rel_linf"] for v in errs.values()),
         "mass_rel_diff": float(abs(mass[-1] - ref_mass[-1]) / abs(ref_mass[-1])),
@@ -218,11 +243,51 @@ def ensemble_leg(args):
     return out
 
 
-def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
-    """The oracle on this host's cores, from the same initial state, for as
-    many steps as fit in about budget_s seconds (at least one)."""
+# omp3's dataflow moves 95 field sweeps per cell-update (760 B in FP64): the
+# CPU baseline's effective bandwidth is stated at that figure.
+B_OMP3_FP64 = 760
+
+
+def host_topology():
+    """This process's CPUs: the affinity mask, and the physical cores,
+    sockets and NUMA nodes behind it (lscpu -p)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    info = {"affinity_cpus": len(allowed), "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET,NODE"], capture_output=True,
+                             text=True, timeout=30).stdout
+        rows = [tuple(int(x) if x else -1 for x in line.split(","))
+                for line in out.splitlines() if line and not line.startswith("#")]
+        mine = [r for r in rows if r[0] in set(allowed)]
+        info["sockets"] = len({r[2] for r in rows})
+        info["physical_cores"] = len({(r[2], r[1]) for r in rows})
+        info["numa_nodes"] = len({r[3] for r in rows})
+        info["affinity_physical_cores"] = len({(r[2], r[1]) for r in mine})
+        info["affinity_numa_nodes"] = len({r[3] for r in mine})
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def oracle_threads():
+    """OpenMP threads and binding for the oracle: OMP_NUM_THREADS if set,
+    else one per CPU of the affinity mask; cores placed and spread, unless
+    the environment says otherwise. Set before the oracle's OpenMP runtime
+    starts (it reads them once, when the library loads)."""
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or len(os.sched_getaffinity(0)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    return threads
+
+
+def oracle_sample(n, dt_param, rho0, e0, budget_s, max_steps=None):
+    """The oracle from an initial state through main.c's loop (the pre-loop
+    set_timestep untimed, then solve_hydro_2d + print_conservation per step,
+    timed as main.c:80-93 times them) for as many steps as fit in about
+    budget_s seconds (at least one, at most max_steps). Returns
+    (cell-updates/s, steps, wall seconds, arrays)."""
     from oracle import pyoracle as po
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     m = po.Mesh(n, n, dt=dt_param)
     a = po.new_arrays(m)
     a["rho"][:] = rho0
@@ -236,16 +301,70 @@ def cpu_baseline(n, seed, dt_param, budget_s, rho0, e0):
         po.conservation(m, a)
         wall += time.perf_counter() - t0
         steps += 1
+        if max_steps and steps >= max_steps:
+            break
         if wall >= budget_s or wall / steps * (steps + 1) > 2.0 * budget_s:
             break
+    return n * n * steps / wall, steps, wall, a
+
+
+def cpu_baseline(n, dt_param, budget_s, rho0, e0):
+    """The oracle on this host's cores, from the multi-blast initial state,
+    for as many steps as fit in about budget_s seconds."""
+    threads = oracle_threads()
+    cups, steps, wall, a = oracle_sample(n, dt_param, rho0, e0, budget_s)
     return {
-        "value": n * n * steps / wall, "unit": "cell-updates/s",
+        "value": cups, "unit": "cell-updates/s",
         "cores": threads, "kind": "port",
         "sample": f"oracle (C+OpenMP restatement of omp3/flow.c) on the "
                   f"{n}x{n} multi-blast, first {steps} steps "
-                  f"({wall:.1f} s, {threads} threads)",
+                  f"({wall:.1f} s, {threads} OpenMP threads, "
+                  f"OMP_PLACES={os.environ['OMP_PLACES']} "
+                  f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']})",
         "steps": steps, "wall_s": wall,
+        "effective_GBps_at_760B": cups * B_OMP3_FP64 / 1e9,
     }, a, steps
+
+
+def other_configs(args):
+    """The reference's CPU configuration (Sod 512^2, 100 steps, .params dt
+    1e-4) and the 4096^2 blast (500 steps, .params dt 1e-4): the GPU over
+    the whole run (interface-driven, as main.c drives it) and the oracle on
+    the same host over config 1 whole and a bounded sample of config 2."""
+    import flow_amd
+    from flow_amd import native, problems
+    oracle_threads()
+    out = {}
+    for label, fn, n, steps, budget in (("sod_512_100_steps", problems.sod_x, 512, 100, 30.0),
+                                        ("blast_4096_500_steps", problems.blast_4096, 4096,
+                                         500, 3.0)):
+        rho0, e0 = fn(n)
+        mesh = flow_amd.Mesh(n, n, dt=1.0e-4)
+        st = flow_amd.DeviceState(mesh)
+        reset_state(st, rho0, e0, mesh, 1.0e-4)
+        st.run(min(steps, 20))                      # warm-up
+        reset_state(st, rho0, e0, mesh, 1.0e-4)
+        st.set_timestep(0)
+        native.sync()
+        t0 = time.perf_counter()
+        done = st.run(steps, pre_timestep=False)[0]
+        native.sync()
+        gpu_s = time.perf_counter() - t0
+        und = st.undefined
+        st.close()
+        mesh.close()
+        cups, csteps, cwall, _ = oracle_sample(n, 1.0e-4, rho0, e0, budget,
+                                               max_steps=steps)
+        out[label] = {
+            "cells": n * n, "steps": done,
+            "gpu_cell_updates_per_s": n * n * done / gpu_s,
+            "gpu_ms_per_step": gpu_s / done * 1e3,
+            "steps_with_undefined_cfl_candidates": int((und[1:] > 0).sum()),
+            "cpu_cell_updates_per_s": cups,
+            "cpu_sample": f"{csteps} steps in {cwall:.2f} s",
+            "cpu_effective_GBps_at_760B": cups * B_OMP3_FP64 / 1e9,
+        }
+    return out
 
 
 def main():
@@ -343,9 +462,8 @@ def main():
                    "cells": n * n, "schedule": sched_name, "graphs": args.graphs,
                    "dt_param": args.dt},
         "roofline": roofline,
-        "step_roofline": {"b_alg_per_cell_update": B_ALG_FP64,
-                          "achieved_GBps": cups * B_ALG_FP64 / 1e9,
-                          "frac": cups * B_ALG_FP64 / HBM_PEAK},
+        "step_roofline": step_roofline(cups, elapsed / steps, n, sched_name, "march",
+                                       B_ALG_FP64, B_MARCH_FP64),
         "kernels": kernels,
         "graph_replay" if not args.graphs else "direct_launch": other_timing,
         "fp32_state": fp32,
@@ -353,10 +471,11 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        base, ora, k = cpu_baseline(n, args.seed, args.dt, args.cpu_seconds,
-                                    rho0, e0)
+        base, ora, k = cpu_baseline(n, args.dt, args.cpu_seconds, rho0, e0)
         result["cpu_baseline"] = {kk: base[kk] for kk in
-                                  ("value", "unit", "cores", "kind", "sample")}
+                                  ("value", "unit", "cores", "kind", "sample",
+                                   "effective_GBps_at_760B")}
+        result["cpu_baseline"]["host"] = host_topology()
         # parity at full size on the same sample: k GPU steps vs k oracle steps
         reset_state(st, rho0, e0, mesh, args.dt)
         got_steps = st.run(k)[0]
@@ -373,6 +492,8 @@ def main():
     st.close()
     mesh.close()
     result["ensemble"] = None if args.no_ensemble else ensemble_leg(args)
+    if not (args.no_other_configs or args.no_cpu_baseline):
+        result["other_configs"] = other_configs(args)
     print(json.dumps(result), flush=True)
 
 
