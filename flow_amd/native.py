@@ -70,6 +70,7 @@ def lib():
                                      ctypes.POINTER(ctypes.c_long)]),
         "flowhip_timing_reset": (i, [mp]),
         "flowhip_run": (i, [mp, ctypes.POINTER(vp), i, i, i, vp, vp, vp, _dp, vp]),
+        "flowhip_solve_hydro_2d_static": (i, [mp, i] + [vp] * 16 + [_dp, _dp, vp]),
         "flowhip_set_cfl_rule": (i, [mp, i]),
         "flowhip_get_cfl_rule": (i, [mp]),
         "flowhip_undefined_candidates": (i, [mp]),
@@ -257,6 +258,15 @@ class DeviceState:
         lib().solve_hydro_2d(self.mesh.ptr, tt,
                              *[ctypes.c_void_p(self.ptrs[k]) for k in ARRAY_NAMES],
                              None)
+
+    def solve_static(self, tt, stream=None):
+        """One step through flowhip_solve_hydro_2d_static (no allocation).
+        Returns (status, mass, energy)."""
+        mass, energy = ctypes.c_double(), ctypes.c_double()
+        rc = lib().flowhip_solve_hydro_2d_static(
+            self.mesh.ptr, tt, *[ctypes.c_void_p(self.ptrs[k]) for k in ARRAY_NAMES],
+            ctypes.byref(mass), ctypes.byref(energy), stream)
+        return rc, mass.value, energy.value
 
     def set_timestep(self, first_step):
         s = self.mesh.s

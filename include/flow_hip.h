@@ -158,6 +158,33 @@ int flowhip_ensemble_run(FlowHipEnsemble* ens, int niters, int tt0,
                          int pre_timestep, double* dt_pairs, double* mass,
                          double* energy, double* wall_seconds, int* undefined);
 
+/* ---- one step without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): one
+ * solve_hydro_2d step (omp3/flow.c:10-52) in the march schedule, bit for
+ * bit the step solve_hydro_2d takes, plus that step's totals (what
+ * print_conservation prints) in *mass and *energy (either may be NULL).
+ * It calls no hipMalloc, hipFree or other allocating routine and creates no
+ * per-Mesh state: its scratch (reduction partials, counters, dt pair,
+ * totals, reciprocal spacings) is one static device array of 1008896 bytes
+ * per GPU, held until the process ends. Meshes up to FLOWHIP_MAX_CELLS_SIDE
+ * interior cells a side; the CFL rule is FLOWHIP_CFL_DEFINED. Reads
+ * mesh->dt and dt_h, writes the next pair into them (as solve_hydro_2d
+ * does), and works on `stream`, a hipStream_t (NULL: the default stream),
+ * synchronising it before it returns. Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (other threads, other
+ * streams) race.
+ * Returns 0; -1 for bad arguments, -2 for a mesh beyond the scratch, -3 if
+ * the scratch cannot be found, -4 for a non-finite or non-positive new dt
+ * (each with a message on stderr). */
+int flowhip_solve_hydro_2d_static(Mesh* mesh, int tt, double* P, double* rho,
+                                  double* rho_old, double* e, double* u, double* v,
+                                  double* mom_x, double* mom_y, double* Qxx,
+                                  double* Qyy, double* F_x, double* F_y,
+                                  double* uF_x, double* uF_y, double* vF_x,
+                                  double* vF_y, double* mass, double* energy,
+                                  void* stream);
+
 /* Device helpers for callers that manage their own buffers. */
 int flowhip_device_sync(void);
 int flowhip_memset_zero(double* dev, size_t len);

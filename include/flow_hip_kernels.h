@@ -314,6 +314,17 @@ hipError_t fk_ens_step(FkEnsLaunch L, int tt, int fp32, hipStream_t s);
 hipError_t fk_ens_timestep(FkEnsLaunch L, int fp32, hipStream_t s);
 hipError_t fk_ens_set_counter(int* counter, int value, hipStream_t s);
 
+/* ---- the no-allocation step (noalloc.hip) ----
+ * fk_static_scratch fills *r and the two reciprocal-spacing buffers (L+1
+ * doubles each) from the current device's static scratch array, whose
+ * device address it looks up once per device; max_side is the largest
+ * interior side those buffers serve, bytes the array's size. Every caller
+ * on one device gets the same memory. fk_static_begin zeroes r's counter
+ * words (ticket, status, queue, totals) and sets r.dtp = {dt, dt_h}. */
+hipError_t fk_static_scratch(FkReduce* r, double** inv_edgedx, double** inv_edgedy,
+                             int* max_side, size_t* bytes);
+hipError_t fk_static_begin(FkReduce r, double dt, double dt_h, hipStream_t s);
+
 /* dst[k] = (float)src[k] / (double)src[k], k < n (round to nearest). */
 hipError_t fk_narrow(long n, float* dst, const double* src, hipStream_t s);
 hipError_t fk_widen(long n, double* dst, const float* src, hipStream_t s);
