@@ -248,10 +248,15 @@ def ensemble_leg(args):
 B_OMP3_FP64 = 760
 
 
+# The affinity mask as the process started: once the oracle's OpenMP runtime
+# loads with OMP_PROC_BIND set, it binds the calling thread to one place.
+START_AFFINITY = sorted(os.sched_getaffinity(0))
+
+
 def host_topology():
-    """This process's CPUs: the affinity mask, and the physical cores,
-    sockets and NUMA nodes behind it (lscpu -p)."""
-    allowed = sorted(os.sched_getaffinity(0))
+    """This process's CPUs: the affinity mask it started with, and the
+    physical cores, sockets and NUMA nodes behind it (lscpu -p)."""
+    allowed = START_AFFINITY
     info = {"affinity_cpus": len(allowed), "logical_cpus": os.cpu_count()}
     try:
         out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET,NODE"], capture_output=True,
@@ -276,7 +281,7 @@ def oracle_threads():
     starts (it reads them once, when the library loads)."""
     os.environ.setdefault("OMP_PLACES", "cores")
     os.environ.setdefault("OMP_PROC_BIND", "spread")
-    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or len(os.sched_getaffinity(0)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or len(START_AFFINITY))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     return threads
 

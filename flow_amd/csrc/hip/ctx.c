@@ -96,6 +96,10 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
    * meshes (the results are the same bits either way) */
   const char* p2 = getenv("FLOWHIP_POW2");
   h->no_pow2 = (p2 && !strcmp(p2, "0")) ? 1 : 0;
+  /* FLOWHIP_UNIFORM=0 keeps the per-row geometry loads on uniform meshes
+   * (the same bits either way) */
+  const char* un = getenv("FLOWHIP_UNIFORM");
+  h->no_uniform = (un && !strcmp(un, "0")) ? 1 : 0;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   live_add(h);
@@ -150,8 +154,19 @@ static int all_pow2(const double* dev, int n, int emax) {
   return ok;
 }
 
+/* Whether all n values are the same. */
+static int all_equal(const double* dev, int n) {
+  double* host = (double*)malloc(sizeof(double) * (size_t)n);
+  if (!host) TERMINATE("hip kernel set: out of host memory");
+  HIPCHECK(hipMemcpy(host, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  int ok = 1;
+  for (int k = 1; k < n && ok; ++k) ok = (host[k] == host[0]);
+  free(host);
+  return ok;
+}
+
 /* 1/edgedx and 1/edgedy, recomputed when the geometry arrays change, and
- * whether the spacings are powers of two. */
+ * whether the spacings are powers of two and uniform. */
 static void ensure_reciprocals(FlowHipCtx* h, const Mesh* mesh) {
   const int lx = mesh->local_nx + 1, ly = mesh->local_ny + 1;
   if (h->inv_src_x == mesh->edgedx && h->inv_src_y == mesh->edgedy &&
@@ -166,6 +181,8 @@ static void ensure_reciprocals(FlowHipCtx* h, const Mesh* mesh) {
   h->pow2_f32 = h->pow2 && all_pow2(mesh->edgedx, lx, 40) &&
                 all_pow2(mesh->edgedy, ly, 40) && all_pow2(mesh->celldx, lx, 40) &&
                 all_pow2(mesh->celldy, ly, 40);
+  h->uniform = all_equal(mesh->edgedx, lx) && all_equal(mesh->edgedy, ly) &&
+               all_equal(mesh->celldx, lx) && all_equal(mesh->celldy, ly);
   h->inv_src_cx = mesh->celldx;
   h->inv_src_cy = mesh->celldy;
   hipFree(h->inv_edgedx);
@@ -188,9 +205,11 @@ FkMesh flowhip_fkmesh(const Mesh* mesh) {
     m.inv_edgedx = h->inv_edgedx;
     m.inv_edgedy = h->inv_edgedy;
     m.pow2 = h->pow2 && !h->no_pow2;
+    m.uniform = m.pow2 && h->uniform && !h->no_uniform;
   } else {
     m.inv_edgedx = m.inv_edgedy = NULL;
     m.pow2 = 0;
+    m.uniform = 0;
   }
   m.nx = mesh->local_nx;
   m.ny = mesh->local_ny;
@@ -357,6 +376,7 @@ FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
   f.inv_edgedx = h->geo32 + 2 * (lx + ly);
   f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
   f.pow2 = h->pow2_f32 && !h->no_pow2;
+  f.uniform = f.pow2 && h->uniform && !h->no_uniform;
   return f;
 }
 

@@ -48,7 +48,9 @@ typedef struct FlowHipCtx {
   const double *inv_src_x, *inv_src_y, *inv_src_cx, *inv_src_cy;
   int inv_nx, inv_ny;
   int pow2, pow2_f32; /* every spacing a power of two (FP64 / FP32 range) */
+  int uniform;        /* each spacing array holds one value throughout */
   int no_pow2;        /* keep the quotients anyway */
+  int no_uniform;     /* keep the per-row geometry loads anyway */
   /* FP32 copies of the geometry for the precision form: edgedx, edgedy,
    * celldx, celldy, 1/edgedx, 1/edgedy, each (L+1) floats */
   float* geo32;

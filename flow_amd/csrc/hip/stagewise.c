@@ -152,7 +152,7 @@ static int same_step(const FlowHipStep* a, const FlowHipStep* b) {
          a->m.edgedx == b->m.edgedx && a->m.edgedy == b->m.edgedy &&
          a->m.celldx == b->m.celldx && a->m.celldy == b->m.celldy &&
          a->m.inv_edgedx == b->m.inv_edgedx && a->m.inv_edgedy == b->m.inv_edgedy &&
-         a->m.pow2 == b->m.pow2;
+         a->m.pow2 == b->m.pow2 && a->m.uniform == b->m.uniform;
 }
 
 void flowhip_enqueue_step(FlowHipCtx* h, const FlowHipStep* st) {

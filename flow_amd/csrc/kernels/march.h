@@ -369,6 +369,37 @@ struct RowRecip {
   }
 };
 
+/* Row geometry of a uniform mesh (each spacing array holds one value
+ * throughout): the value is read once per work item and every row reads it
+ * from a scalar register. Same values as RowScalar / RowRecip at every row.
+ * Measured on MI355X at 8192^2: the fused momentum march 5% faster, the
+ * mass/energy march 0.5-1%; the Lagrangian march was not (it keeps the
+ * per-row loads). */
+template <typename T>
+struct RowConst {
+  T v;
+  __device__ RowConst(const T* arr, int, int, int, bool = false)
+      : v(((const __attribute__((address_space(4))) T*)(unsigned long long)arr)[0]) {}
+  __device__ __forceinline__ T operator()(int) const { return v; }
+};
+template <typename T, bool P2>
+struct RowConstRecip {
+  T v;
+  __device__ RowConstRecip(const T* arr, int, int, int, bool = true) {
+    const T d = ((const __attribute__((address_space(4))) T*)(unsigned long long)arr)[0];
+    if constexpr (P2)
+      v = recip_pow2(d);
+    else
+      v = T(1.0) / d;
+  }
+  __device__ __forceinline__ T operator()(int) const { return v; }
+};
+/* UNI selects the uniform-mesh forms */
+template <typename T, bool UNI>
+using RowS = typename std::conditional<UNI, RowConst<T>, RowScalar<T>>::type;
+template <typename T, bool P2, bool UNI>
+using RowR = typename std::conditional<UNI, RowConstRecip<T, P2>, RowRecip<T, P2>>::type;
+
 /* ---- boundary-strip writers (ghost writes only in edge items) ---- */
 template <typename T>
 __device__ __forceinline__ void mput_cell(bool edge, T* a, int nx, int ny, int i,
@@ -422,7 +453,7 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * [x0, x1] and the first (x) update of cells [x0, x1); per lane a window of
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
-template <typename T, bool P2, bool EDGE = true>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false>
 __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -444,11 +475,11 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   const T cdx_q = g.cdx[qc], edx_q = g.edx[qc], edx_q1 = g.edx[qc + 1];
   const int qg = face ? q : x0;
   const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
-  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* rows of the x-updated fields for this lane's column: w[0] oldest */
   T wr[4], we[4];
@@ -549,7 +580,7 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -563,9 +594,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
     if (item_edge(ml, g.nx, g.ny, item))
-      mass_xy_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+      mass_xy_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
     else
-      mass_xy_item<T, P2, false>(g, io, dt, dt_h, ml, item, lane, ms, es);
+      mass_xy_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -581,7 +612,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * column (the same column inside the domain). Per row r: the y flux at face
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
-template <typename T, bool P2, bool EDGE = true>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false>
 __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -604,11 +635,11 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   const T cdx_q = g.cdx[qc];
   const int qg = face ? q : x0;
   const T cdx_qm1 = g.cdx[qg - 1], iedx_q = g.iedx[qg];
-  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q, rcdx_m = T(1.0) / cdx_m;
-  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2 of column qm */
   T wr[4], we[4], wv[3];
@@ -694,7 +725,7 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -708,9 +739,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
     if (item_edge(ml, g.nx, g.ny, item))
-      mass_yx_item<T, P2>(g, io, dt, dt_h, ml, item, lane, ms, es);
+      mass_yx_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
     else
-      mass_yx_item<T, P2, false>(g, io, dt, dt_h, ml, item, lane, ms, es);
+      mass_yx_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -1513,7 +1544,7 @@ struct MMom2IO {
  * columns: the y stage's corner flux at face x1 needs u at x1, hence the x
  * stage's cell flux at x1 and u0 up to x1+2). The per-row arithmetic is the
  * two separate kernels', so the results are theirs bit for bit. */
-template <typename T, bool P2, int XE = -1, bool EDGE = true>
+template <typename T, bool P2, int XE = -1, bool EDGE = true, bool UNI = false>
 __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
@@ -1540,11 +1571,11 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
   const bool own_y = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
   const T iedx_q = g.iedx[qg];
-  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* x stage: u1 and m1 of cell row i from that row's u0, F_x, mx1, rho */
   auto xrow = [&](int i, T a_u, T a_fx, T a_m1, T a_r, T& m1, T& u1) {
@@ -1709,7 +1740,7 @@ __device__ __forceinline__ void momxy_xy_item(const MGeo<T>& g, const MMom2IO<T>
  * flux in x) and y momentum (y flux in y, v recompute, y flux in x) in one
  * march; the y stage at row r reads u of rows r and r-1 from the x stage.
  * Lane l holds x face / column q = x0 - 3 + l (strips of 59 columns). */
-template <typename T, bool P2, int XE = -1, bool EDGE = true>
+template <typename T, bool P2, int XE = -1, bool EDGE = true, bool UNI = false>
 __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>& io,
                                               T dt, T dt_h, const MarchList& ml,
                                               int item, int lane) {
@@ -1735,11 +1766,11 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   const bool own_y = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
   const T iedx_q = g.iedx[qg];
-  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
-  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T rcdx_q = T(1.0) / cdx_q;
-  const RowRecip<T, P2> rcdy(g.cdy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
 
   /* x stage: x flux in y at corner row gf from u0 rows gf-2 .. gf+1 */
   auto corner = [&](int gf, T w0, T w1, T w2, T w3, T a_v, T a_fy) {
@@ -1864,7 +1895,7 @@ __device__ __forceinline__ void momxy_yx_item(const MGeo<T>& g, const MMom2IO<T>
   }
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
@@ -1875,16 +1906,16 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_xy(
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
     if (!item_edge(ml, g.nx, g.ny, item))
-      momxy_xy_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane);
+      momxy_xy_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, item, lane);
     else if (item_xedge(ml, g.nx, item))
-      momxy_xy_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      momxy_xy_item<T, P2, 1, true, UNI>(g, io, dt, dt_h, ml, item, lane);
     else
-      momxy_xy_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+      momxy_xy_item<T, P2, 0, true, UNI>(g, io, dt, dt_h, ml, item, lane);
   }
   mq.finish();
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     MGeo<T> g, MMom2IO<T> io, const double* dtp, unsigned int* queue, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
@@ -1895,11 +1926,11 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momxy_yx(
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
     if (!item_edge(ml, g.nx, g.ny, item))
-      momxy_yx_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane);
+      momxy_yx_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, item, lane);
     else if (item_xedge(ml, g.nx, item))
-      momxy_yx_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane);
+      momxy_yx_item<T, P2, 1, true, UNI>(g, io, dt, dt_h, ml, item, lane);
     else
-      momxy_yx_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane);
+      momxy_yx_item<T, P2, 0, true, UNI>(g, io, dt, dt_h, ml, item, lane);
   }
   mq.finish();
 }

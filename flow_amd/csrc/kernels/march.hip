@@ -56,20 +56,22 @@ inline MarchList march_items(int nx, int ny, int W, int chunk_rows, Taper t) {
 
 /* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
 constexpr int kMassW = kWave - 4;
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
   const MarchList ml = march_items(g.nx, g.ny, kMassW, chunk_rows, taper_of<T>(kKMass));
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_xy<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_mass_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
+    const int grid =
+        march_grid(k_march_mass_xy<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_mass_xy<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_mass_yx<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_mass_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
+    const int grid =
+        march_grid(k_march_mass_yx<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_mass_yx<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g, io,
                        (const double*)r.dtp, r, ml);
   }
   return hipGetLastError();
@@ -122,32 +124,37 @@ hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
 /* both momentum directions in one march: strips of 59 columns */
 constexpr int kMomXYW = kWave - 5;
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
                                 unsigned int* queue, int chunk_rows, hipStream_t s) {
   const MarchList ml = march_items(g.nx, g.ny, kMomXYW, chunk_rows, taper_of<T>(kKMom));
   if (ml.nitems < 1 || !queue) return hipErrorInvalidValue;
   if (tt % 2) {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_xy<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_momxy_xy<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       dtp, queue, ml);
+    const int grid =
+        march_grid(k_march_momxy_xy<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_momxy_xy<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g,
+                       io, dtp, queue, ml);
   } else {
     static int per_cu = 0;
-    const int grid = march_grid(k_march_momxy_yx<T, P2>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_momxy_yx<T, P2>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       dtp, queue, ml);
+    const int grid =
+        march_grid(k_march_momxy_yx<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_momxy_yx<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g,
+                       io, dtp, queue, ml);
   }
   return hipGetLastError();
 }
 
 /* Dispatch on whether every mesh spacing is a power of two (quotients by
- * spacings become products with exact reciprocals). */
+ * spacings become products with exact reciprocals) and, for the mass/energy
+ * and fused momentum marches, whether such a mesh is also uniform (row
+ * geometry read once per item). */
 template <typename T>
 hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
-                             int chunk_rows, int pow2, hipStream_t s) {
-  return pow2 ? launch_march_mass_p<T, true>(g, io, tt, r, chunk_rows, s)
-              : launch_march_mass_p<T, false>(g, io, tt, r, chunk_rows, s);
+                             int chunk_rows, int pow2, int uniform, hipStream_t s) {
+  if (pow2 && uniform) return launch_march_mass_p<T, true, true>(g, io, tt, r, chunk_rows, s);
+  return pow2 ? launch_march_mass_p<T, true, false>(g, io, tt, r, chunk_rows, s)
+              : launch_march_mass_p<T, false, false>(g, io, tt, r, chunk_rows, s);
 }
 
 template <typename T>
@@ -170,10 +177,12 @@ hipError_t launch_march_momentum(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
 
 template <typename T>
 hipError_t launch_march_momxy(MGeo<T> g, MMom2IO<T> io, int tt, const double* dtp,
-                              unsigned int* queue, int chunk_rows, int pow2,
+                              unsigned int* queue, int chunk_rows, int pow2, int uniform,
                               hipStream_t s) {
-  return pow2 ? launch_march_momxy_p<T, true>(g, io, tt, dtp, queue, chunk_rows, s)
-              : launch_march_momxy_p<T, false>(g, io, tt, dtp, queue, chunk_rows, s);
+  if (pow2 && uniform)
+    return launch_march_momxy_p<T, true, true>(g, io, tt, dtp, queue, chunk_rows, s);
+  return pow2 ? launch_march_momxy_p<T, true, false>(g, io, tt, dtp, queue, chunk_rows, s)
+              : launch_march_momxy_p<T, false, false>(g, io, tt, dtp, queue, chunk_rows, s);
 }
 
 }  // namespace flowhip
@@ -205,7 +214,7 @@ hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v
                              int chunk_rows, hipStream_t s) {
   MMom2IO<double> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
   return launch_march_momxy<double>(mgeo64(m), io, tt, r.dtp, r.queue, chunk_rows,
-                                    m.pow2, s);
+                                    m.pow2, m.uniform, s);
 }
 
 hipError_t fk_march_momentum_x(FkMesh m, int tt, const double* u0,
@@ -232,7 +241,8 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s) {
   MMassIO<double> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, s);
+  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, m.uniform,
+                                   s);
 }
 
 static MGeo<float> mgeo32(FkMesh32 m) {
@@ -258,7 +268,7 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
                                     float* F_x, float* F_y, FkReduce r,
                                     int chunk_rows, hipStream_t s) {
   MMassIO<float> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, s);
+  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, m.uniform, s);
 }
 
 hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,
@@ -268,7 +278,7 @@ hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const floa
                                  int chunk_rows, hipStream_t s) {
   MMom2IO<float> io{u0, v0, F_x, F_y, rho, mx1, my1, mom_x, mom_y, u, v};
   return launch_march_momxy<float>(mgeo32(m), io, tt, r.dtp, r.queue, chunk_rows,
-                                   m.pow2, s);
+                                   m.pow2, m.uniform, s);
 }
 
 hipError_t fk_march_momentum_x_f32(FkMesh32 m, int tt, const float* u0,

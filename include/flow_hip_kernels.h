@@ -30,6 +30,10 @@ typedef struct FkMesh {
   /* 1 when every spacing above is a power of two (the march kernels then
    * divide by spacings as products with exact reciprocals), else 0 */
   int pow2;
+  /* 1 when, besides, each of the four spacing arrays holds one value
+   * throughout (the mass/energy and momentum marches then read the row
+   * geometry once per work item instead of once per row), else 0 */
+  int uniform;
 } FkMesh;
 
 /* Geometry of the FP32-state precision form: the same arrays rounded to
@@ -39,6 +43,7 @@ typedef struct FkMesh32 {
   const float *edgedx, *edgedy, *celldx, *celldy;
   const float *inv_edgedx, *inv_edgedy;
   int pow2;
+  int uniform; /* as in FkMesh */
 } FkMesh32;
 
 /* Device scratch for the single-pass grid reductions. */
