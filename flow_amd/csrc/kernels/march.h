@@ -189,13 +189,15 @@ struct MarchQueue {
   }
 };
 
-/* Block form of the queue, for the fused momentum marches: thread 0 claims
- * kMWaves consecutive items of the band for the whole block and wave w
- * marches base + w, so a block's waves march adjacent strips of one chunk
- * row side by side and share the halo columns between them in cache (the
- * other marches ran slower this way, waiting at the barriers). Every wave
- * of the block takes part in every claim, and the loop ends for the whole
- * block at once:
+/* Block form of the queue, for the Lagrangian and fused momentum marches:
+ * thread 0 claims kMWaves consecutive items of the band for the whole block
+ * and wave w marches base + w, so a block's waves march adjacent strips of
+ * one chunk row side by side, share the halo columns between them in cache
+ * and read and write each row's neighbouring segments together (measured on
+ * MI355X at 8192^2: the Lagrangian march 2.5-3% faster than with per-wave
+ * claims; the mass/energy marches 1% slower, waiting at the barriers, so
+ * they keep the per-wave queue). Every wave of the block takes part in
+ * every claim, and the loop ends for the whole block at once:
  *
  *   MarchBlockQueue mq(queue, nitems);
  *   for (int base = mq.next(); base < mq.end; base = mq.next()) {
@@ -1502,8 +1504,11 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   constexpr bool kSq = std::is_same<T, float>::value;
   double local_min = kSq ? max_dt * max_dt : max_dt;
   int nan_count = 0;
-  MarchQueue<true> mq(red.queue, ml.nitems);
-  for (int item = mq.next(); item < mq.end; item = mq.next()) {
+  MarchBlockQueue mq(red.queue, ml.nitems);
+  const int wave = threadIdx.x / kWave;
+  for (int base = mq.next(); base < mq.end; base = mq.next()) {
+    const int item = __builtin_amdgcn_readfirstlane(base + wave);
+    if (item >= mq.end) continue;
     if (!item_edge(ml, g.nx, g.ny, item))
       lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane, local_min,
                                        nan_count);
