@@ -63,6 +63,10 @@ constexpr int kMWaves = kMNT / kWave;
  * slower. */
 template <typename T>
 constexpr int kMarchMaxWaves = sizeof(T) == 8 ? 3 : 5;
+/* The FP32 Lagrangian march: four. In its uniform-mesh form (95 VGPRs) five
+ * would fit, and made it 8% slower. */
+template <typename T>
+constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 4;
 
 /* The work items of a march kernel: strips of W columns by chunks of rows.
  * The interior rows are cut into G row bands (one per XCD band of the
@@ -1278,7 +1282,7 @@ struct MLagIO {
  * EDGE = false is the interior form, for items that item_edge clears: no
  * ghost store, mirror or wall test is compiled, so each row is one
  * straight-line block. */
-template <typename T, bool P2, int XE = -1, bool EDGE = true>
+template <typename T, bool P2, int XE = -1, bool EDGE = true, bool UNI = false>
 __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, double& local_min, int& nan_count) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
@@ -1301,13 +1305,15 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   const bool own_face = q >= x0 && (q < x1 || (q == x1 && x1 == nx - kPad));
   const bool own = q >= x0 && q < x1;
   const T edx_q = g.edx[qg], cdx_q = g.cdx[qg], cdx_qm = g.cdx[qg - 1];
-  const RowScalar<T> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   /* reciprocal spacings, used in place of quotients on power-of-two meshes */
   const T iedx_q = g.iedx[qg], rcdx_q = T(1.0) / cdx_q;
-  const RowScalar<T> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   /* lane-held reciprocals in FP32: 2% faster there than RowRecip */
-  using Recip = typename std::conditional<std::is_same<T, float>::value, RowGeo<T>,
-                                          RowRecip<T, P2>>::type;
+  using Recip = typename std::conditional<
+      UNI, RowConstRecip<T, P2>,
+      typename std::conditional<std::is_same<T, float>::value, RowGeo<T>,
+                                RowRecip<T, P2>>::type>::type;
   const Recip rcdy(g.cdy, y0 - 4, ny, lane, true);
 
   /* y face k of the lane's column after pressure acceleration, from cell
@@ -1494,7 +1500,7 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   }
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
                                                            const double* dtp,
                                                            FkReduce red, double max_dt,
@@ -1510,12 +1516,12 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
     if (!item_edge(ml, g.nx, g.ny, item))
-      lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, item, lane, local_min,
+      lagrangian_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, item, lane, local_min,
                                        nan_count);
     else if (item_xedge(ml, g.nx, item))
-      lagrangian_item<T, P2, 1>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+      lagrangian_item<T, P2, 1, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
     else
-      lagrangian_item<T, P2, 0>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+      lagrangian_item<T, P2, 0, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
   }
   mq.finish();
   finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);
