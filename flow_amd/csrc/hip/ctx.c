@@ -100,6 +100,10 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
    * (the same bits either way) */
   const char* un = getenv("FLOWHIP_UNIFORM");
   h->no_uniform = (un && !strcmp(un, "0")) ? 1 : 0;
+  /* FLOWHIP_FP32_PAIRS=0 keeps one column per lane in the FP32 Lagrangian
+   * and mass/energy marches (the same fields either way) */
+  const char* pr = getenv("FLOWHIP_FP32_PAIRS");
+  h->no_pairs = (pr && !strcmp(pr, "0")) ? 1 : 0;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   live_add(h);
@@ -377,6 +381,7 @@ FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
   f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
   f.pow2 = h->pow2_f32 && !h->no_pow2;
   f.uniform = f.pow2 && h->uniform && !h->no_uniform;
+  f.pairs = !h->no_pairs;
   return f;
 }
 

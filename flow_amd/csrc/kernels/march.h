@@ -67,6 +67,17 @@ constexpr int kMarchMaxWaves = sizeof(T) == 8 ? 3 : 5;
  * would fit, and made it 8% slower. */
 template <typename T>
 constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 4;
+/* The FP32 marches with two columns per lane: three (the Lagrangian's 124
+ * VGPRs and the mass/energy march's 107-109 allow four, 3-7% slower at
+ * every chunk height measured; two waves for the Lagrangian: 3% slower). */
+#ifndef FLOWHIP_LAG_PAIR_WAVES
+#define FLOWHIP_LAG_PAIR_WAVES 3
+#endif
+#ifndef FLOWHIP_MASS_PAIR_WAVES
+#define FLOWHIP_MASS_PAIR_WAVES 3
+#endif
+constexpr int kLagPairWaves = FLOWHIP_LAG_PAIR_WAVES;
+constexpr int kMassPairWaves = FLOWHIP_MASS_PAIR_WAVES;
 
 /* The work items of a march kernel: strips of W columns by chunks of rows.
  * The interior rows are cut into G row bands (one per XCD band of the
@@ -451,6 +462,64 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
   if (j == nx - kPad - 2) a[g * nx + nx - kPad + 1] = v;
 }
 
+/* ---- two columns per lane: the FP32 mass/energy and Lagrangian marches'
+ * interior items ----
+ * Lane l holds columns qb + c, c = 0, 1, with qb = x0 - 2 + 2l (mass/energy:
+ * 124 owned columns, two halo columns on each side) or x0 - 4 + 2l
+ * (Lagrangian: 120 owned, four halo columns each side). A row of any array
+ * is one 8-byte access per lane (512 bytes per wave, as a row of the FP64
+ * march), and the row's addresses, lane shifts, row tests and loop control
+ * serve two columns. Each column's arithmetic is the one-column march's, in
+ * the same order, so the fields are bitwise the same. */
+constexpr int kMassW = kWave - 4;       /* one column per lane */
+constexpr int kMassW2 = 2 * kWave - 4;  /* two */
+typedef float MF2 __attribute__((ext_vector_type(2)));
+/* the same at 4-byte alignment: the x-face arrays' rows (odd stride nx+1)
+ * start on odd columns every other row; global_load_dwordx2 and
+ * global_store_dwordx2 need only dword alignment */
+typedef MF2 MF2u __attribute__((aligned(4)));
+
+/* the value at column qb + c + d of a pair array, d in -2..2.
+ * Take every shift in wave-uniform control flow, before any branch on the
+ * lane's columns: a lane shift reads zero from a lane the branch has
+ * switched off. */
+template <typename T>
+__device__ __forceinline__ T pshift(const T (&x)[2], int c, int d) {
+  const int k = c + d;
+  if (k < 0) return dpp_move<kDppWaveShr1>(x[k + 2]);
+  if (k > 1) return dpp_move<kDppWaveShl1>(x[k - 2]);
+  return x[k];
+}
+/* a row's two columns of a cell or y-face array (8-byte aligned) */
+__device__ __forceinline__ void ld_pair(const float* a, long idx, float (&x)[2]) {
+  const MF2 v = *reinterpret_cast<const MF2*>(a + idx);
+  x[0] = v.x;
+  x[1] = v.y;
+}
+__device__ __forceinline__ void st_pair(float* a, long idx, const float (&x)[2]) {
+  MF2 v;
+  v.x = x[0];
+  v.y = x[1];
+  *reinterpret_cast<MF2*>(a + idx) = v;
+}
+/* the same for an x-face array (4-byte aligned) */
+__device__ __forceinline__ void ld_pair4(const float* a, long idx, float (&x)[2]) {
+  const MF2 v = *reinterpret_cast<const MF2u*>(a + idx);
+  x[0] = v.x;
+  x[1] = v.y;
+}
+__device__ __forceinline__ void st_pair4(float* a, long idx, const float (&x)[2]) {
+  MF2 v;
+  v.x = x[0];
+  v.y = x[1];
+  *reinterpret_cast<MF2u*>(a + idx) = v;
+}
+template <typename T>
+__device__ __forceinline__ void copy_pair(T (&d)[2], const T (&s)[2]) {
+  d[0] = s[0];
+  d[1] = s[1];
+}
+
 /* ======================================================================= */
 /* mass and energy, x sweep first (even tt)                                 */
 /* ======================================================================= */
@@ -460,11 +529,10 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
 template <typename T, bool P2, bool EDGE = true, bool UNI = false>
-__device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
+__device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const ItemBox box = item_box(ml, nx, item);
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
   if (y0 >= y1) return;
   const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
@@ -586,7 +654,156 @@ __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T, bool P2, bool UNI = false>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false>
+__device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane,
+                                             CSum& ms, CSum& es) {
+  mass_xy_box<T, P2, EDGE, UNI>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
+}
+
+/* The x-first mass/energy march of an interior FP32 item, two columns per
+ * lane (mass_xy_box's interior form, column by column). */
+template <bool P2, bool UNI>
+__device__ __forceinline__ void mass_xy_pair(const MGeo<float>& g, const MMassIO<float>& io,
+                                             float dt, float dt_h, const ItemBox box, int lane,
+                                             CSum& ms, CSum& es) {
+  using T = float;
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const int qb = x0 - 2 + 2 * lane; /* the lane's first column */
+  /* x0 and the strip width are even, so a lane owns both columns or neither */
+  const bool own = qb >= x0 && qb < x1;
+  bool face[2];
+  T cdx_q[2], edx_q[2], edx_q1[2], cdx_qm1[2], iedx_q[2], rcdx_q[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int q = qb + c;
+    face[c] = q >= x0 && q <= x1;
+    cdx_q[c] = g.cdx[q];
+    edx_q[c] = g.edx[q];
+    edx_q1[c] = g.edx[q + 1];
+    const int qg = face[c] ? q : x0;
+    cdx_qm1[c] = g.cdx[qg - 1];
+    iedx_q[c] = g.iedx[qg];
+    rcdx_q[c] = T(1.0) / cdx_q[c];
+  }
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
+  auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
+  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_pair4(a, (long)row * sx + qb, x); };
+
+  /* windows: [row][column], row 0 oldest */
+  T wr[4][2], we[4][2], vy[3][2], r0[3][2];
+  T fy_prev[2] = {T(0), T(0)}, efy_prev[2] = {T(0), T(0)};
+  const int rb = y0 - 2;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    wr[k][0] = wr[k][1] = we[k][0] = we[k][1] = T(0);
+  ldc(io.v0, rb - 2, vy[0]);
+  ldc(io.v0, rb - 1, vy[1]);
+  ldc(io.rho0, rb - 2, r0[0]);
+  ldc(io.rho0, rb - 1, r0[1]);
+  T a_r[2], a_e[2], a_u[2], a_v[2];
+  ldc(io.rho0, rb, a_r);
+  ldc(io.e1, rb, a_e);
+  ldx(io.u0, rb, a_u);
+  ldc(io.v0, rb, a_v);
+  for (int r = rb; r <= y1 + 1; ++r) {
+    /* the next row (an interior item's rows all lie inside the mesh) */
+    T n_r[2], n_e[2], n_u[2], n_v[2];
+    ldc(io.rho0, r + 1, n_r);
+    ldc(io.e1, r + 1, n_e);
+    ldx(io.u0, r + 1, n_u);
+    ldc(io.v0, r + 1, n_v);
+    copy_pair(vy[2], a_v);
+    copy_pair(r0[2], a_r);
+
+    /* x fluxes and the x update of row r */
+    T fm[2], fe[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T um1 = pshift(a_u, c, -1), up1 = pshift(a_u, c, 1);
+      const T rm2 = pshift(a_r, c, -2), rm1 = pshift(a_r, c, -1), rp1 = pshift(a_r, c, 1);
+      const T em2 = pshift(a_e, c, -2), em1 = pshift(a_e, c, -1), ep1 = pshift(a_e, c, 1);
+      FaceFlux<T> fl{T(0), T(0)};
+      if (face[c])
+        fl = mass_energy_flux<T, false, P2>(um1, a_u[c], up1, rm2, rm1, a_r[c], rp1, em2,
+                                            em1, a_e[c], ep1, iedx_q[c], cdx_qm1[c],
+                                            cdx_q[c], edy(r), dt, dt_h, rcdx_q[c]);
+      fm[c] = fl.mass;
+      fe[c] = fl.energy;
+    }
+    if (own && r >= y0 && r < y1) st_pair4(io.F_x, (long)r * sx + qb, fm);
+    T xr[2], xe[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const CellState<T> st =
+          sweep_update<T, P2>(a_r[c], a_r[c], a_e[c], fm[c], pshift(fm, c, 1), fe[c],
+                              pshift(fe, c, 1), edy(r), edy(r + 1), cdx_q[c], cdy(r), dt_h,
+                              true, rcdx_q[c] * rcdy(r));
+      xr[c] = st.rho;
+      xe[c] = st.e;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      copy_pair(wr[k], wr[k + 1]);
+      copy_pair(we[k], we[k + 1]);
+    }
+    copy_pair(wr[3], xr);
+    copy_pair(we[3], xe);
+    /* y flux at face r-1 from rows r-3 .. r, and the y update of row r-2 */
+    const int gf = r - 1;
+    if (gf >= y0 && gf <= y1) {
+      T fy[2], efy[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
+            vy[0][c], vy[1][c], vy[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c],
+            we[0][c], we[1][c], we[2][c], we[3][c], iedy(gf), cdy(gf - 1), cdy(gf),
+            edx_q[c], dt, dt_h, rcdy(gf));
+        fy[c] = fl.mass;
+        efy[c] = fl.energy;
+      }
+      if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fy);
+      const int i = r - 2;
+      if (i >= y0 && own) {
+        T nr[2], ne[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const CellState<T> st = sweep_update<T, P2>(
+              wr[1][c], r0[0][c], we[1][c], fy_prev[c], fy[c], efy_prev[c], efy[c],
+              edx_q[c], edx_q1[c], cdx_q[c], cdy(i), dt_h, false, rcdx_q[c] * rcdy(i));
+          nr[c] = st.rho;
+          ne[c] = st.e;
+        }
+        st_pair(io.rho, (long)i * nx + qb, nr);
+        st_pair(io.e, (long)i * nx + qb, ne);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          ms.add(double(nr[c]));
+          es.add(double(nr[c]) * double(ne[c]));
+        }
+      }
+      copy_pair(fy_prev, fy);
+      copy_pair(efy_prev, efy);
+    }
+    copy_pair(vy[0], vy[1]);
+    copy_pair(vy[1], vy[2]);
+    copy_pair(r0[0], r0[1]);
+    copy_pair(r0[1], r0[2]);
+    copy_pair(a_r, n_r);
+    copy_pair(a_e, n_e);
+    copy_pair(a_u, n_u);
+    copy_pair(a_v, n_v);
+  }
+}
+
+/* PAIR: interior items two columns per lane (FP32, strips of kMassW2
+ * columns; edge items go through the one-column form in sub-strips of
+ * kMassW). */
+template <typename T, bool P2, bool UNI = false, bool PAIR = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -599,10 +816,21 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
-    if (item_edge(ml, g.nx, g.ny, item))
+    if constexpr (PAIR) {
+      const ItemBox b = item_box(ml, g.nx, item);
+      if (item_edge(ml, g.nx, g.ny, item)) {
+        for (int sx0 = b.x0; sx0 < b.x1; sx0 += kMassW) {
+          const ItemBox sb{sx0, min(sx0 + kMassW, b.x1), b.y0, b.y1};
+          mass_xy_box<T, P2, true, UNI>(g, io, dt, dt_h, sb, lane, ms, es);
+        }
+      } else {
+        mass_xy_pair<P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
+      }
+    } else if (item_edge(ml, g.nx, g.ny, item)) {
       mass_xy_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
-    else
+    } else {
       mass_xy_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    }
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -619,11 +847,10 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
 template <typename T, bool P2, bool EDGE = true, bool UNI = false>
-__device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, CSum& ms, CSum& es) {
+__device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, CSum& ms, CSum& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const ItemBox box = item_box(ml, nx, item);
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
   if (y0 >= y1) return;
   const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
@@ -731,7 +958,160 @@ __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>&
   }
 }
 
-template <typename T, bool P2, bool UNI = false>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false>
+__device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane,
+                                             CSum& ms, CSum& es) {
+  mass_yx_box<T, P2, EDGE, UNI>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
+}
+
+/* The y-first mass/energy march of an interior FP32 item, two columns per
+ * lane (mass_yx_box's interior form, column by column). */
+template <bool P2, bool UNI>
+__device__ __forceinline__ void mass_yx_pair(const MGeo<float>& g, const MMassIO<float>& io,
+                                             float dt, float dt_h, const ItemBox box, int lane,
+                                             CSum& ms, CSum& es) {
+  using T = float;
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const int qb = x0 - 2 + 2 * lane;
+  const bool own = qb >= x0 && qb < x1;
+  bool face[2];
+  T cdx_q[2], edx_q[2], edx_q1[2], cdx_qm1[2], iedx_q[2], rcdx_q[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int q = qb + c;
+    face[c] = q >= x0 && q <= x1;
+    cdx_q[c] = g.cdx[q];
+    edx_q[c] = g.edx[q];
+    edx_q1[c] = g.edx[q + 1];
+    const int qg = face[c] ? q : x0;
+    cdx_qm1[c] = g.cdx[qg - 1];
+    iedx_q[c] = g.iedx[qg];
+    rcdx_q[c] = T(1.0) / cdx_q[c];
+  }
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
+  const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
+  auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
+  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_pair4(a, (long)row * sx + qb, x); };
+
+  /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2: [row][column] */
+  T wr[4][2], we[4][2], wv[3][2];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ldc(io.rho0, y0 - 2 + k, wr[k]);
+    ldc(io.e1, y0 - 2 + k, we[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) ldc(io.v0, y0 - 1 + k, wv[k]);
+  /* y flux at face y0 (rows y0-2 .. y0+1) */
+  T fpm[2], fpe[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
+        wv[0][c], wv[1][c], wv[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c], we[0][c],
+        we[1][c], we[2][c], we[3][c], iedy(y0), cdy(y0 - 1), cdy(y0), edx_q[c], dt, dt_h,
+        rcdy(y0));
+    fpm[c] = fl.mass;
+    fpe[c] = fl.energy;
+  }
+  if (own) st_pair(io.F_y, (long)y0 * nx + qb, fpm);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    copy_pair(wr[k], wr[k + 1]);
+    copy_pair(we[k], we[k + 1]);
+  }
+  copy_pair(wv[0], wv[1]);
+  copy_pair(wv[1], wv[2]);
+  T n_r[2], n_e[2], n_v[2], a_u[2], a_r0[2];
+  ldc(io.rho0, y0 + 2, n_r);
+  ldc(io.e1, y0 + 2, n_e);
+  ldc(io.v0, y0 + 2, n_v);
+  ldx(io.u0, y0, a_u);
+  ldc(io.rho0, y0, a_r0);
+  for (int r = y0; r < y1; ++r) {
+    copy_pair(wr[3], n_r);
+    copy_pair(we[3], n_e);
+    copy_pair(wv[2], n_v);
+    T c_u[2], c_r0[2];
+    copy_pair(c_u, a_u);
+    copy_pair(c_r0, a_r0);
+    /* the next row's inputs (an interior item's rows all lie inside) */
+    ldc(io.rho0, r + 3, n_r);
+    ldc(io.e1, r + 3, n_e);
+    ldc(io.v0, r + 3, n_v);
+    ldx(io.u0, r + 1, a_u);
+    ldc(io.rho0, r + 1, a_r0);
+    /* y flux at face r+1 (rows r-1 .. r+2) and the y update of row r */
+    const int gf = r + 1;
+    T fm[2], fe[2], yr[2], ye[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
+          wv[0][c], wv[1][c], wv[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c], we[0][c],
+          we[1][c], we[2][c], we[3][c], iedy(gf), cdy(gf - 1), cdy(gf), edx_q[c], dt, dt_h,
+          rcdy(gf));
+      fm[c] = fl.mass;
+      fe[c] = fl.energy;
+      const CellState<T> ys =
+          sweep_update<T, P2>(wr[1][c], wr[1][c], we[1][c], fpm[c], fl.mass, fpe[c],
+                              fl.energy, edx_q[c], edx_q1[c], cdx_q[c], cdy(r), dt_h, true,
+                              rcdx_q[c] * rcdy(r));
+      yr[c] = ys.rho;
+      ye[c] = ys.e;
+    }
+    if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fm);
+    copy_pair(fpm, fm);
+    copy_pair(fpe, fe);
+    /* x fluxes of the y-updated row, then the x update */
+    T xm[2], xe[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T um1 = pshift(c_u, c, -1), up1 = pshift(c_u, c, 1);
+      const T rm2 = pshift(yr, c, -2), rm1 = pshift(yr, c, -1), rp1 = pshift(yr, c, 1);
+      const T em2 = pshift(ye, c, -2), em1 = pshift(ye, c, -1), ep1 = pshift(ye, c, 1);
+      FaceFlux<T> fx{T(0), T(0)};
+      if (face[c])
+        fx = mass_energy_flux<T, false, P2>(um1, c_u[c], up1, rm2, rm1, yr[c], rp1, em2, em1,
+                                            ye[c], ep1, iedx_q[c], cdx_qm1[c], cdx_q[c],
+                                            edy(r), dt, dt_h, rcdx_q[c]);
+      xm[c] = fx.mass;
+      xe[c] = fx.energy;
+    }
+    if (own) st_pair4(io.F_x, (long)r * sx + qb, xm);
+    const T xm_hi[2] = {pshift(xm, 0, 1), pshift(xm, 1, 1)};
+    const T xe_hi[2] = {pshift(xe, 0, 1), pshift(xe, 1, 1)};
+    if (own) {
+      T nr[2], ne[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const CellState<T> st = sweep_update<T, P2>(
+            yr[c], c_r0[c], ye[c], xm[c], xm_hi[c], xe[c], xe_hi[c], edy(r), edy(r + 1),
+            cdx_q[c], cdy(r), dt_h, false, rcdx_q[c] * rcdy(r));
+        nr[c] = st.rho;
+        ne[c] = st.e;
+      }
+      st_pair(io.rho, (long)r * nx + qb, nr);
+      st_pair(io.e, (long)r * nx + qb, ne);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        ms.add(double(nr[c]));
+        es.add(double(nr[c]) * double(ne[c]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      copy_pair(wr[k], wr[k + 1]);
+      copy_pair(we[k], we[k + 1]);
+    }
+    copy_pair(wv[0], wv[1]);
+    copy_pair(wv[1], wv[2]);
+  }
+}
+
+template <typename T, bool P2, bool UNI = false, bool PAIR = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
                                                         const double* dtp,
                                                         FkReduce red,
@@ -744,10 +1124,21 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
-    if (item_edge(ml, g.nx, g.ny, item))
+    if constexpr (PAIR) {
+      const ItemBox b = item_box(ml, g.nx, item);
+      if (item_edge(ml, g.nx, g.ny, item)) {
+        for (int sx0 = b.x0; sx0 < b.x1; sx0 += kMassW) {
+          const ItemBox sb{sx0, min(sx0 + kMassW, b.x1), b.y0, b.y1};
+          mass_yx_box<T, P2, true, UNI>(g, io, dt, dt_h, sb, lane, ms, es);
+        }
+      } else {
+        mass_yx_pair<P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
+      }
+    } else if (item_edge(ml, g.nx, g.ny, item)) {
       mass_yx_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
-    else
+    } else {
       mass_yx_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
+    }
     fm.add(ms.s + ms.c);
     fe.add(es.s + es.c);
   }
@@ -1283,11 +1674,12 @@ struct MLagIO {
  * ghost store, mirror or wall test is compiled, so each row is one
  * straight-line block. */
 template <typename T, bool P2, int XE = -1, bool EDGE = true, bool UNI = false>
-__device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane, double& local_min, int& nan_count) {
+__device__ __forceinline__ void lagrangian_box(const MGeo<T>& g, const MLagIO<T>& io, T dt,
+                                               T dt_h, const ItemBox box, int lane,
+                                               double& local_min, int& nan_count) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
-  const ItemBox box = item_box(ml, nx, item);
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
   if (y0 >= y1) return;
   const bool edge = EDGE && ((x0 == kPad) || (x1 + 3 > nx - kPad) || (y0 == kPad) ||
@@ -1500,7 +1892,216 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
   }
 }
 
-template <typename T, bool P2, bool UNI = false>
+template <typename T, bool P2, int XE = -1, bool EDGE = true, bool UNI = false>
+__device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T>& io, T dt,
+                                                T dt_h, const MarchList& ml, int item,
+                                                int lane, double& local_min, int& nan_count) {
+  lagrangian_box<T, P2, XE, EDGE, UNI>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane,
+                                       local_min, nan_count);
+}
+
+/* ---- the Lagrangian march, two columns per lane (FP32 interior items) ----
+ * Lane l holds columns qb + c, c = 0, 1, with qb = x0 - 4 + 2l: strips of
+ * 120 columns with four halo columns on each side (the one-column form
+ * needs three). Every row access is one 8-byte access per lane. Each
+ * column's arithmetic is lagrangian_box's interior form in the same order,
+ * so the fields are bitwise the same; every lane shift is taken in
+ * wave-uniform control flow. */
+constexpr int kLagW = kWave - 6;       /* one column per lane */
+constexpr int kLagW2 = 2 * kWave - 8;  /* two */
+
+template <bool P2, bool UNI>
+__device__ __forceinline__ void lagrangian_pair(const MGeo<float>& g, const MLagIO<float>& io,
+                                                float dt, float dt_h, const ItemBox box,
+                                                int lane, double& local_min, int& nan_count) {
+  using T = float;
+  using P = T[2];
+  const int nx = g.nx, ny = g.ny, sx = nx + 1;
+  const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
+  if (y0 >= y1) return;
+  const int qb = x0 - 4 + 2 * lane;
+  const bool own = qb >= x0 && qb < x1; /* both columns: x0 and the width even */
+  T edx_q[2], cdx_q[2], cdx_qm[2], iedx_q[2], rcdx_q[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int q = qb + c;
+    edx_q[c] = g.edx[q];
+    cdx_q[c] = g.cdx[q];
+    cdx_qm[c] = g.cdx[q - 1];
+    iedx_q[c] = g.iedx[q];
+    rcdx_q[c] = T(1.0) / cdx_q[c];
+  }
+  const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
+  const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
+  using Recip = typename std::conditional<UNI, RowConstRecip<T, P2>, RowGeo<T>>::type;
+  const Recip rcdy(g.cdy, y0 - 4, ny, lane, true);
+  auto ldc = [&](const T* a, int row, P& x) { ld_pair(a, (long)row * nx + qb, x); };
+  auto ldx = [&](const T* a, int row, P& x) { ld_pair4(a, (long)row * sx + qb, x); };
+
+  /* y face k after pressure acceleration, from cell rows k (rk, ek) and
+   * k-1 (rs, es) and mom_y at face k */
+  auto ypa = [&](int k, const P& rk, const P& ek, const P& rs, const P& es, const P& my,
+                 P& mom, P& rey, P& v) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      mom[c] = accelerate<T, P2>(my[c], dt, eos_pressure(rk[c], ek[c]),
+                                 eos_pressure(rs[c], es[c]), edy(k), iedy(k));
+      rey[c] = rho_edge_y<T, P2>(rk[c], rs[c], cdx_q[c], cdy(k), cdy(k - 1), edy(k),
+                                 T(0.5) * rcdx_q[c] * iedy(k));
+      v[c] = velocity(mom[c], rey[c]);
+    }
+  };
+
+  /* ---- prologue: y faces y0-2 .. y0+2 and v after the viscous step at y0 */
+  T wr[4][2], we[4][2];
+  T vp[5][2], pm[5][2], pr[5][2];
+  T qy_prev[2], vav_prev[2];
+  {
+    T rr[6][2], ee[6][2];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      ldc(io.rho, y0 - 3 + s, rr[s]);
+      ldc(io.e, y0 - 3 + s, ee[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      T my[2];
+      ldc(io.mom_y, y0 - 2 + s, my);
+      ypa(y0 - 2 + s, rr[s + 1], ee[s + 1], rr[s], ee[s], my, pm[s], pr[s], vp[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      copy_pair(wr[s], rr[s + 3]);
+      copy_pair(we[s], ee[s + 3]);
+    }
+    /* Qyy at cell rows y0 and y0-1, v after the viscous step at face y0 */
+    T mom0[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      qy_prev[c] = viscous_stress(rr[3][c], vp[1][c], vp[2][c], vp[3][c], vp[4][c]);
+      const T qy_m1 = viscous_stress(rr[2][c], vp[0][c], vp[1][c], vp[2][c], vp[3][c]);
+      mom0[c] = accelerate<T, P2>(pm[2][c], dt, qy_prev[c], qy_m1, cdy(y0), rcdy(y0));
+      vav_prev[c] = velocity(mom0[c], pr[2][c]);
+    }
+    if (own) {
+      st_pair(io.my1, (long)y0 * nx + qb, mom0);
+      st_pair(io.v0, (long)y0 * nx + qb, vav_prev);
+    }
+  }
+  /* windows entering row y0: v at faces y0 .. y0+2; mom, rho edge at faces
+   * y0+1, y0+2 */
+  T wv[4][2], wm[3][2], wx[3][2];
+  copy_pair(wv[0], vp[2]);
+  copy_pair(wv[1], vp[3]);
+  copy_pair(wv[2], vp[4]);
+  copy_pair(wm[0], pm[3]);
+  copy_pair(wm[1], pm[4]);
+  copy_pair(wx[0], pr[3]);
+  copy_pair(wx[1], pr[4]);
+  ldc(io.rho, y0 + 3, wr[3]);
+  ldc(io.e, y0 + 3, we[3]);
+  T a_my[2], a_mx[2], a_r[2], a_e[2];
+  ldc(io.mom_y, y0 + 3, a_my);
+  ldx(io.mom_x, y0, a_mx);
+  ldc(io.rho, y0 + 4, a_r);
+  ldc(io.e, y0 + 4, a_e);
+
+  for (int i = y0; i < y1; ++i) {
+    T c_my[2], c_mx[2], n_r[2], n_e[2];
+    copy_pair(c_my, a_my);
+    copy_pair(c_mx, a_mx);
+    copy_pair(n_r, a_r);
+    copy_pair(n_e, a_e);
+    /* the next rows (an interior item's rows all lie inside the mesh) */
+    ldc(io.mom_y, i + 4, a_my);
+    ldx(io.mom_x, i + 1, a_mx);
+    ldc(io.rho, i + 5, a_r);
+    ldc(io.e, i + 5, a_e);
+    /* y face i+3 after pressure acceleration */
+    ypa(i + 3, wr[3], we[3], wr[2], we[2], c_my, wm[2], wx[2], wv[3]);
+    /* Qyy at cell row i+1, then y face i+1 after the viscous acceleration */
+    T qy[2], my_av[2], vav[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      qy[c] = viscous_stress(wr[1][c], wv[0][c], wv[1][c], wv[2][c], wv[3][c]);
+      my_av[c] = accelerate<T, P2>(wm[0][c], dt, qy[c], qy_prev[c], cdy(i + 1), rcdy(i + 1));
+      vav[c] = velocity(my_av[c], wx[0][c]);
+    }
+    if (own && i + 1 < y1) {
+      st_pair(io.my1, (long)(i + 1) * nx + qb, my_av);
+      st_pair(io.v0, (long)(i + 1) * nx + qb, vav);
+    }
+
+    /* along x on row i */
+    T Pr[2], mx_pa[2], rex[2], u_pa[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) Pr[c] = eos_pressure(wr[0][c], we[0][c]);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      mx_pa[c] = accelerate<T, P2>(c_mx[c], dt, Pr[c], pshift(Pr, c, -1), edx_q[c],
+                                   iedx_q[c]);
+      rex[c] = rho_edge_x<T, P2>(wr[0][c], pshift(wr[0], c, -1), cdx_q[c], cdx_qm[c], cdy(i),
+                                 edx_q[c], T(0.5) * iedx_q[c] * rcdy(i));
+      u_pa[c] = velocity(mx_pa[c], rex[c]);
+    }
+    T qx[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      qx[c] = viscous_stress(wr[0][c], pshift(u_pa, c, -1), u_pa[c], pshift(u_pa, c, 1),
+                             pshift(u_pa, c, 2));
+    T mx_av[2], uav[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      mx_av[c] = accelerate<T, P2>(mx_pa[c], dt, qx[c], pshift(qx, c, -1), cdx_q[c],
+                                   rcdx_q[c]);
+      uav[c] = velocity(mx_av[c], rex[c]);
+    }
+    const T uav_e[2] = {pshift(uav, 0, 1), pshift(uav, 1, 1)};
+    if (own) {
+      st_pair4(io.mx1, (long)i * sx + qb, mx_av);
+      st_pair4(io.u0, (long)i * sx + qb, uav);
+      /* shock heating, CFL candidate and the copy of rho on row i */
+      T es[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        es[c] = shock_heating<T, P2>(we[0][c], Pr[c], wr[0][c], qx[c], qy_prev[c], uav[c],
+                                     uav_e[c], vav_prev[c], vav[c], cdx_q[c], cdy(i), dt_h,
+                                     rcdx_q[c], rcdy(i));
+      st_pair(io.e1, (long)i * nx + qb, es);
+      st_pair(io.rho_old, (long)i * nx + qb, wr[0]);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        /* FP64 CFL candidate, squared */
+        const double cand = cell_timestep_sq(double(es[c]), double(qx[c]),
+                                             double(qy_prev[c]), double(wr[0][c]),
+                                             double(cdx_q[c]), double(cdy(i)));
+        if (cand == cand)
+          local_min = lesser(local_min, cand);
+        else
+          ++nan_count;
+        if (io.cfl) io.cfl[(long)i * nx + qb + c] = cand;
+      }
+    }
+
+    /* advance the windows */
+    copy_pair(qy_prev, qy);
+    copy_pair(vav_prev, vav);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      copy_pair(wv[s], wv[s + 1]);
+      copy_pair(wr[s], wr[s + 1]);
+      copy_pair(we[s], we[s + 1]);
+    }
+    copy_pair(wr[3], n_r);
+    copy_pair(we[3], n_e);
+    copy_pair(wm[0], wm[1]);
+    copy_pair(wm[1], wm[2]);
+    copy_pair(wx[0], wx[1]);
+    copy_pair(wx[1], wx[2]);
+  }
+}
+
+template <typename T, bool P2, bool UNI = false, bool PAIR = false>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
                                                            const double* dtp,
                                                            FkReduce red, double max_dt,
@@ -1515,13 +2116,29 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
   for (int base = mq.next(); base < mq.end; base = mq.next()) {
     const int item = __builtin_amdgcn_readfirstlane(base + wave);
     if (item >= mq.end) continue;
-    if (!item_edge(ml, g.nx, g.ny, item))
+    if constexpr (PAIR) {
+      /* interior items two columns per lane; edge items in one-column
+       * sub-strips */
+      const ItemBox b = item_box(ml, g.nx, item);
+      if (!item_edge(ml, g.nx, g.ny, item)) {
+        lagrangian_pair<P2, UNI>(g, io, dt, dt_h, b, lane, local_min, nan_count);
+      } else {
+        for (int sx0 = b.x0; sx0 < b.x1; sx0 += kLagW) {
+          const ItemBox sb{sx0, min(sx0 + kLagW, b.x1), b.y0, b.y1};
+          lagrangian_box<T, P2, -1, true, UNI>(g, io, dt, dt_h, sb, lane, local_min,
+                                               nan_count);
+        }
+      }
+    } else if (!item_edge(ml, g.nx, g.ny, item)) {
       lagrangian_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, item, lane, local_min,
-                                       nan_count);
-    else if (item_xedge(ml, g.nx, item))
-      lagrangian_item<T, P2, 1, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
-    else
-      lagrangian_item<T, P2, 0, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min, nan_count);
+                                            nan_count);
+    } else if (item_xedge(ml, g.nx, item)) {
+      lagrangian_item<T, P2, 1, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min,
+                                           nan_count);
+    } else {
+      lagrangian_item<T, P2, 0, true, UNI>(g, io, dt, dt_h, ml, item, lane, local_min,
+                                           nan_count);
+    }
   }
   mq.finish();
   finish_timestep<kMNT, kSq>(local_min, nan_count, red, max_dt, first_step);

@@ -10,7 +10,9 @@ namespace flowhip {
  * units of H2 rows, chunks of k units, and nsmall one-unit chunks at the end
  * of each of the 8 XCD row bands (MarchList). Measured on MI355X at 8192^2,
  * uniform chunks (k = 1, nsmall = 0) of 24 / 20 / 16 rows (Lagrangian /
- * mass-energy / momentum; 24 / 16 / 20 in FP32) are fastest: tapering from
+ * mass-energy / momentum; 24 / 16 / 20 in FP32, and 10 for the FP32
+ * Lagrangian and mass/energy marches with two columns per lane) are
+ * fastest: tapering from
  * tall chunks (48-80 rows) down to 12-24-row ones cut the re-read prologue
  * rows but made the Lagrangian and mass/energy marches 5-10% slower. An
  * explicit chunk_rows > 0 gives uniform chunks of that height. Every
@@ -36,6 +38,16 @@ struct Taper {
 #ifndef FLOWHIP_TAPER_MOM32
 #define FLOWHIP_TAPER_MOM32 20, 1, 0
 #endif
+/* the FP32 mass/energy march with two columns per lane (three waves per
+ * SIMD): 10 rows (measured 8 to 32; 14 rows +0.3%, 20 +3%) */
+#ifndef FLOWHIP_TAPER_MASS32P
+#define FLOWHIP_TAPER_MASS32P 10, 1, 0
+#endif
+/* the FP32 Lagrangian march with two columns per lane (three waves per
+ * SIMD): 10 rows (measured 4 to 24; 12 rows +0.5%, 16 +4%, 20 +5%) */
+#ifndef FLOWHIP_TAPER_LAG32P
+#define FLOWHIP_TAPER_LAG32P 10, 1, 0
+#endif
 enum { kKLag = 0, kKMass = 1, kKMom = 2 };
 template <typename T>
 constexpr Taper taper_of(int kernel) {
@@ -54,40 +66,61 @@ inline MarchList march_items(int nx, int ny, int W, int chunk_rows, Taper t) {
   return make_march(nx, ny, W, t.H2, k, t.nsmall, kMarchBands);
 }
 
-/* Strip width: the 64 lanes hold the strip and its x halo of 2 + 2 columns. */
-constexpr int kMassW = kWave - 4;
-template <typename T, bool P2, bool UNI>
+template <typename T, bool P2, bool UNI, bool PAIR = false>
 hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
-  const MarchList ml = march_items(g.nx, g.ny, kMassW, chunk_rows, taper_of<T>(kKMass));
+  const MarchList ml = march_items(g.nx, g.ny, PAIR ? kMassW2 : kMassW, chunk_rows,
+                                   PAIR ? Taper{FLOWHIP_TAPER_MASS32P} : taper_of<T>(kKMass));
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
-    const int grid =
-        march_grid(k_march_mass_xy<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_mass_xy<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       (const double*)r.dtp, r, ml);
+    const int grid = march_grid(k_march_mass_xy<T, P2, UNI, PAIR>, per_cu, ml.nitems,
+                                PAIR ? kMassPairWaves : kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_mass_xy<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
+                       g, io, (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
-    const int grid =
-        march_grid(k_march_mass_yx<T, P2, UNI>, per_cu, ml.nitems, kMarchMaxWaves<T>);
-    hipLaunchKernelGGL((k_march_mass_yx<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                       (const double*)r.dtp, r, ml);
+    const int grid = march_grid(k_march_mass_yx<T, P2, UNI, PAIR>, per_cu, ml.nitems,
+                                PAIR ? kMassPairWaves : kMarchMaxWaves<T>);
+    hipLaunchKernelGGL((k_march_mass_yx<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
+                       g, io, (const double*)r.dtp, r, ml);
   }
   return hipGetLastError();
 }
 
-template <typename T, bool P2, bool UNI>
+/* Whether an FP32 mass/energy march can hold two columns per lane: rows of
+ * the cell and y-face arrays it reads or writes as column pairs start on
+ * 8-byte boundaries (nx even, arrays 8-byte aligned; strips start at even
+ * columns). */
+inline bool pairs_fit(const MGeo<float>& g, const MMassIO<float>& io) {
+  const unsigned long long bits =
+      (unsigned long long)io.v0 | (unsigned long long)io.rho0 | (unsigned long long)io.e1 |
+      (unsigned long long)io.rho | (unsigned long long)io.e | (unsigned long long)io.F_y;
+  return g.nx % 2 == 0 && (bits & 7ull) == 0;
+}
+
+template <typename T, bool P2, bool UNI, bool PAIR = false>
 hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                      int first_step, int chunk_rows, hipStream_t s) {
-  const MarchList ml = march_items(g.nx, g.ny, kWave - 6, chunk_rows, taper_of<T>(kKLag));
+  const MarchList ml = march_items(g.nx, g.ny, PAIR ? kLagW2 : kLagW, chunk_rows,
+                                   PAIR ? Taper{FLOWHIP_TAPER_LAG32P} : taper_of<T>(kKLag));
   if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
-  const int grid =
-      march_grid(k_march_lagrangian<T, P2, UNI>, per_cu, ml.nitems, kLagMaxWaves<T>);
-  hipLaunchKernelGGL((k_march_lagrangian<T, P2, UNI>), dim3(grid), dim3(kMNT), 0, s, g, io,
-                     (const double*)r.dtp, r, max_dt, first_step, ml);
+  const int grid = march_grid(k_march_lagrangian<T, P2, UNI, PAIR>, per_cu, ml.nitems,
+                              PAIR ? kLagPairWaves : kLagMaxWaves<T>);
+  hipLaunchKernelGGL((k_march_lagrangian<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
+                     g, io, (const double*)r.dtp, r, max_dt, first_step, ml);
   return hipGetLastError();
+}
+
+/* Whether an FP32 Lagrangian march can hold two columns per lane (as
+ * pairs_fit, for the arrays it reads or writes in column pairs). */
+inline bool lag_pairs_fit(const MGeo<float>& g, const MLagIO<float>& io) {
+  const unsigned long long bits =
+      (unsigned long long)io.rho | (unsigned long long)io.e | (unsigned long long)io.mom_y |
+      (unsigned long long)io.my1 | (unsigned long long)io.v0 | (unsigned long long)io.e1 |
+      (unsigned long long)io.rho_old;
+  return g.nx % 2 == 0 && (bits & 7ull) == 0;
 }
 
 /* Even-tt x momentum needs faces [x0-2, x1+1] and one lane below them. */
@@ -122,6 +155,7 @@ hipError_t launch_march_momentum_p(MGeo<T> g, MMomIO<T> io, int tt, int ydir,
   return hipGetLastError();
 }
 
+
 /* both momentum directions in one march: strips of 59 columns */
 constexpr int kMomXYW = kWave - 5;
 
@@ -152,7 +186,11 @@ hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* 
  * geometry read once per item). */
 template <typename T>
 hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
-                             int chunk_rows, int pow2, int uniform, hipStream_t s) {
+                             int chunk_rows, int pow2, int uniform, int pairs, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (pairs && pow2 && uniform && pairs_fit(g, io))
+      return launch_march_mass_p<T, true, true, true>(g, io, tt, r, chunk_rows, s);
+  }
   if (pow2 && uniform) return launch_march_mass_p<T, true, true>(g, io, tt, r, chunk_rows, s);
   return pow2 ? launch_march_mass_p<T, true, false>(g, io, tt, r, chunk_rows, s)
               : launch_march_mass_p<T, false, false>(g, io, tt, r, chunk_rows, s);
@@ -163,8 +201,11 @@ hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
 template <typename T>
 hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                    int first_step, int chunk_rows, int pow2, int uniform,
-                                   hipStream_t s) {
+                                   int pairs, hipStream_t s) {
   if constexpr (sizeof(T) == 4) {
+    if (pairs && pow2 && uniform && lag_pairs_fit(g, io))
+      return launch_march_lagrangian_p<T, true, true, true>(g, io, r, max_dt, first_step,
+                                                            chunk_rows, s);
     if (pow2 && uniform)
       return launch_march_lagrangian_p<T, true, true>(g, io, r, max_dt, first_step,
                                                       chunk_rows, s);
@@ -212,7 +253,7 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
                                int first_step, int chunk_rows, hipStream_t s) {
   MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old, r.cfl};
   return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
-                                         chunk_rows, m.pow2, m.uniform, s);
+                                         chunk_rows, m.pow2, m.uniform, 0, s);
 }
 
 hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
@@ -249,7 +290,7 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s) {
   MMassIO<double> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, m.uniform,
+  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, m.uniform, 0,
                                    s);
 }
 
@@ -267,7 +308,7 @@ hipError_t fk_march_lagrangian_f32(FkMesh32 m, const float* rho, const float* e,
                                    hipStream_t s) {
   MLagIO<float> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old};
   return launch_march_lagrangian<float>(mgeo32(m), io, r, max_dt, first_step,
-                                        chunk_rows, m.pow2, m.uniform, s);
+                                        chunk_rows, m.pow2, m.uniform, m.pairs, s);
 }
 
 hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
@@ -276,7 +317,8 @@ hipError_t fk_march_mass_energy_f32(FkMesh32 m, int tt, const float* u0,
                                     float* F_x, float* F_y, FkReduce r,
                                     int chunk_rows, hipStream_t s) {
   MMassIO<float> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, m.uniform, s);
+  return launch_march_mass<float>(mgeo32(m), io, tt, r, chunk_rows, m.pow2, m.uniform,
+                                  m.pairs, s);
 }
 
 hipError_t fk_march_momentum_f32(FkMesh32 m, int tt, const float* u0, const float* v0,

@@ -44,6 +44,11 @@ typedef struct FkMesh32 {
   const float *inv_edgedx, *inv_edgedy;
   int pow2;
   int uniform; /* as in FkMesh */
+  /* 1 when the Lagrangian and mass/energy marches may hold two columns
+   * per lane on uniform power-of-two meshes (each does when nx is even and
+   * the cell and y-face arrays it reads and writes are 8-byte aligned),
+   * else 0 */
+  int pairs;
 } FkMesh32;
 
 /* Device scratch for the single-pass grid reductions. */
