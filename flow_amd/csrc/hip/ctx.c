@@ -100,9 +100,9 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
    * (the same bits either way) */
   const char* un = getenv("FLOWHIP_UNIFORM");
   h->no_uniform = (un && !strcmp(un, "0")) ? 1 : 0;
-  /* FLOWHIP_FP32_PAIRS=0 keeps one column per lane in the FP32 Lagrangian
-   * and mass/energy marches (the same fields either way) */
-  const char* pr = getenv("FLOWHIP_FP32_PAIRS");
+  /* FLOWHIP_PAIRS=0 keeps one column per lane in the Lagrangian and
+   * mass/energy marches (the same fields either way) */
+  const char* pr = getenv("FLOWHIP_PAIRS");
   h->no_pairs = (pr && !strcmp(pr, "0")) ? 1 : 0;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
@@ -210,10 +210,12 @@ FkMesh flowhip_fkmesh(const Mesh* mesh) {
     m.inv_edgedy = h->inv_edgedy;
     m.pow2 = h->pow2 && !h->no_pow2;
     m.uniform = m.pow2 && h->uniform && !h->no_uniform;
+    m.pairs = !h->no_pairs;
   } else {
     m.inv_edgedx = m.inv_edgedy = NULL;
     m.pow2 = 0;
     m.uniform = 0;
+    m.pairs = 0;
   }
   m.nx = mesh->local_nx;
   m.ny = mesh->local_ny;

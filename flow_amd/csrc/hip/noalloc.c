@@ -73,6 +73,7 @@ int flowhip_solve_hydro_2d_static(Mesh* mesh, int tt, double* P, double* rho,
   st.m.inv_edgedy = inv_y;
   st.m.pow2 = 0;
   st.m.uniform = 0;
+  st.m.pairs = 0;
   st.tt = tt;
   st.max_dt = mesh->max_dt;
   st.P = P;

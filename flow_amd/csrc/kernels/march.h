@@ -67,17 +67,20 @@ constexpr int kMarchMaxWaves = sizeof(T) == 8 ? 3 : 5;
  * would fit, and made it 8% slower. */
 template <typename T>
 constexpr int kLagMaxWaves = sizeof(T) == 8 ? 3 : 4;
-/* The FP32 marches with two columns per lane: three (the Lagrangian's 124
- * VGPRs and the mass/energy march's 107-109 allow four, 3-7% slower at
- * every chunk height measured; two waves for the Lagrangian: 3% slower). */
+/* The marches with two columns per lane: in FP32 three (the Lagrangian's
+ * 124 VGPRs and the mass/energy march's 107-109 allow four, 3-7% slower at
+ * every chunk height measured; two waves for the Lagrangian: 3% slower);
+ * in FP64 two, all that their 192 and 173-177 VGPRs allow. */
 #ifndef FLOWHIP_LAG_PAIR_WAVES
 #define FLOWHIP_LAG_PAIR_WAVES 3
 #endif
 #ifndef FLOWHIP_MASS_PAIR_WAVES
 #define FLOWHIP_MASS_PAIR_WAVES 3
 #endif
-constexpr int kLagPairWaves = FLOWHIP_LAG_PAIR_WAVES;
-constexpr int kMassPairWaves = FLOWHIP_MASS_PAIR_WAVES;
+template <typename T>
+constexpr int kLagPairWaves = sizeof(T) == 8 ? 2 : FLOWHIP_LAG_PAIR_WAVES;
+template <typename T>
+constexpr int kMassPairWaves = sizeof(T) == 8 ? 2 : FLOWHIP_MASS_PAIR_WAVES;
 
 /* The work items of a march kernel: strips of W columns by chunks of rows.
  * The interior rows are cut into G row bands (one per XCD band of the
@@ -473,11 +476,23 @@ __device__ __forceinline__ void mput_yface(bool edge, T* a, int nx, int ny, int 
  * the same order, so the fields are bitwise the same. */
 constexpr int kMassW = kWave - 4;       /* one column per lane */
 constexpr int kMassW2 = 2 * kWave - 4;  /* two */
-typedef float MF2 __attribute__((ext_vector_type(2)));
-/* the same at 4-byte alignment: the x-face arrays' rows (odd stride nx+1)
- * start on odd columns every other row; global_load_dwordx2 and
- * global_store_dwordx2 need only dword alignment */
-typedef MF2 MF2u __attribute__((aligned(4)));
+/* Two consecutive values of a row: V aligned to its size, for cell and
+ * y-face arrays; Vx aligned to one value, for the x-face arrays, whose rows
+ * (odd stride nx+1) start on odd columns every other row
+ * (global_load_dwordx2/x4 and global_store_dwordx2/x4 need only dword
+ * alignment). */
+template <typename T>
+struct PairVec;
+template <>
+struct PairVec<float> {
+  typedef float V __attribute__((ext_vector_type(2)));
+  typedef V Vx __attribute__((aligned(4)));
+};
+template <>
+struct PairVec<double> {
+  typedef double V __attribute__((ext_vector_type(2)));
+  typedef V Vx __attribute__((aligned(8)));
+};
 
 /* the value at column qb + c + d of a pair array, d in -2..2.
  * Take every shift in wave-uniform control flow, before any branch on the
@@ -490,35 +505,53 @@ __device__ __forceinline__ T pshift(const T (&x)[2], int c, int d) {
   if (k > 1) return dpp_move<kDppWaveShl1>(x[k - 2]);
   return x[k];
 }
-/* a row's two columns of a cell or y-face array (8-byte aligned) */
-__device__ __forceinline__ void ld_pair(const float* a, long idx, float (&x)[2]) {
-  const MF2 v = *reinterpret_cast<const MF2*>(a + idx);
+/* a row's two columns of a cell or y-face array (idx even) */
+template <typename T>
+__device__ __forceinline__ void ld_pair(const T* a, long idx, T (&x)[2]) {
+  const typename PairVec<T>::V v = *reinterpret_cast<const typename PairVec<T>::V*>(a + idx);
   x[0] = v.x;
   x[1] = v.y;
 }
-__device__ __forceinline__ void st_pair(float* a, long idx, const float (&x)[2]) {
-  MF2 v;
+template <typename T>
+__device__ __forceinline__ void st_pair(T* a, long idx, const T (&x)[2]) {
+  typename PairVec<T>::V v;
   v.x = x[0];
   v.y = x[1];
-  *reinterpret_cast<MF2*>(a + idx) = v;
+  *reinterpret_cast<typename PairVec<T>::V*>(a + idx) = v;
 }
-/* the same for an x-face array (4-byte aligned) */
-__device__ __forceinline__ void ld_pair4(const float* a, long idx, float (&x)[2]) {
-  const MF2 v = *reinterpret_cast<const MF2u*>(a + idx);
+/* the same for an x-face array (any idx) */
+template <typename T>
+__device__ __forceinline__ void ld_xpair(const T* a, long idx, T (&x)[2]) {
+  const typename PairVec<T>::V v = *reinterpret_cast<const typename PairVec<T>::Vx*>(a + idx);
   x[0] = v.x;
   x[1] = v.y;
 }
-__device__ __forceinline__ void st_pair4(float* a, long idx, const float (&x)[2]) {
-  MF2 v;
+template <typename T>
+__device__ __forceinline__ void st_xpair(T* a, long idx, const T (&x)[2]) {
+  typename PairVec<T>::V v;
   v.x = x[0];
   v.y = x[1];
-  *reinterpret_cast<MF2u*>(a + idx) = v;
+  *reinterpret_cast<typename PairVec<T>::Vx*>(a + idx) = v;
 }
 template <typename T>
 __device__ __forceinline__ void copy_pair(T (&d)[2], const T (&s)[2]) {
   d[0] = s[0];
   d[1] = s[1];
 }
+
+/* A lane's sums of an item's cells for the mass and energy totals, rounded
+ * once into the exact fixed-point accumulators. FP64 state: compensated
+ * (CSum). FP32 state: rho and rho*e formed in FP64 are exact (24-bit
+ * mantissas, a 48-bit product), and a lane adds at most 2 x 120 of them
+ * per item, so a plain FP64 sum is within 240 x 2^-53 of the exact one
+ * (relative), far inside the FP32 state's own rounding, at 4 FP64
+ * operations a cell where the compensated sums take 17. */
+struct PlainSum {
+  double s, c; /* c stays 0: the same interface as CSum */
+  __device__ void add(double x) { s += x; }
+};
+template <typename T>
+using ItemSum = typename std::conditional<sizeof(T) == 4, PlainSum, CSum>::type;
 
 /* ======================================================================= */
 /* mass and energy, x sweep first (even tt)                                 */
@@ -528,8 +561,8 @@ __device__ __forceinline__ void copy_pair(T (&d)[2], const T (&s)[2]) {
  * [x0, x1] and the first (x) update of cells [x0, x1); per lane a window of
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
-template <typename T, bool P2, bool EDGE = true, bool UNI = false>
-__device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, CSum& ms, CSum& es) {
+template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
+__device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, Acc& ms, Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -654,20 +687,19 @@ __device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& 
   }
 }
 
-template <typename T, bool P2, bool EDGE = true, bool UNI = false>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
 __device__ __forceinline__ void mass_xy_item(const MGeo<T>& g, const MMassIO<T>& io, T dt,
                                              T dt_h, const MarchList& ml, int item, int lane,
-                                             CSum& ms, CSum& es) {
-  mass_xy_box<T, P2, EDGE, UNI>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
+                                             Acc& ms, Acc& es) {
+  mass_xy_box<T, P2, EDGE, UNI, Acc>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
 }
 
-/* The x-first mass/energy march of an interior FP32 item, two columns per
- * lane (mass_xy_box's interior form, column by column). */
-template <bool P2, bool UNI>
-__device__ __forceinline__ void mass_xy_pair(const MGeo<float>& g, const MMassIO<float>& io,
-                                             float dt, float dt_h, const ItemBox box, int lane,
-                                             CSum& ms, CSum& es) {
-  using T = float;
+/* The x-first mass/energy march of an interior item, two columns per lane
+ * (mass_xy_box's interior form, column by column). */
+template <typename T, bool P2, bool UNI, typename Acc>
+__device__ __forceinline__ void mass_xy_pair(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                             T dt_h, const ItemBox box, int lane, Acc& ms,
+                                             Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
   if (y0 >= y1) return;
@@ -692,7 +724,7 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<float>& g, const MMassIO
   const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
   auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
-  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_pair4(a, (long)row * sx + qb, x); };
+  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_xpair(a, (long)row * sx + qb, x); };
 
   /* windows: [row][column], row 0 oldest */
   T wr[4][2], we[4][2], vy[3][2], r0[3][2];
@@ -735,7 +767,7 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<float>& g, const MMassIO
       fm[c] = fl.mass;
       fe[c] = fl.energy;
     }
-    if (own && r >= y0 && r < y1) st_pair4(io.F_x, (long)r * sx + qb, fm);
+    if (own && r >= y0 && r < y1) st_xpair(io.F_x, (long)r * sx + qb, fm);
     T xr[2], xe[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -815,16 +847,16 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
   FixSum fm{0, 0, 0, 0}, fe{0, 0, 0, 0};
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
-    CSum ms{0.0, 0.0}, es{0.0, 0.0};
+    ItemSum<T> ms{0.0, 0.0}, es{0.0, 0.0};
     if constexpr (PAIR) {
       const ItemBox b = item_box(ml, g.nx, item);
       if (item_edge(ml, g.nx, g.ny, item)) {
         for (int sx0 = b.x0; sx0 < b.x1; sx0 += kMassW) {
           const ItemBox sb{sx0, min(sx0 + kMassW, b.x1), b.y0, b.y1};
-          mass_xy_box<T, P2, true, UNI>(g, io, dt, dt_h, sb, lane, ms, es);
+          mass_xy_box<T, P2, true, UNI, ItemSum<T>>(g, io, dt, dt_h, sb, lane, ms, es);
         }
       } else {
-        mass_xy_pair<P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
+        mass_xy_pair<T, P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
       }
     } else if (item_edge(ml, g.nx, g.ny, item)) {
       mass_xy_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
@@ -846,8 +878,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * column (the same column inside the domain). Per row r: the y flux at face
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
-template <typename T, bool P2, bool EDGE = true, bool UNI = false>
-__device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, CSum& ms, CSum& es) {
+template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
+__device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, Acc& ms, Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -958,20 +990,19 @@ __device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& 
   }
 }
 
-template <typename T, bool P2, bool EDGE = true, bool UNI = false>
+template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
 __device__ __forceinline__ void mass_yx_item(const MGeo<T>& g, const MMassIO<T>& io, T dt,
                                              T dt_h, const MarchList& ml, int item, int lane,
-                                             CSum& ms, CSum& es) {
-  mass_yx_box<T, P2, EDGE, UNI>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
+                                             Acc& ms, Acc& es) {
+  mass_yx_box<T, P2, EDGE, UNI, Acc>(g, io, dt, dt_h, item_box(ml, g.nx, item), lane, ms, es);
 }
 
-/* The y-first mass/energy march of an interior FP32 item, two columns per
- * lane (mass_yx_box's interior form, column by column). */
-template <bool P2, bool UNI>
-__device__ __forceinline__ void mass_yx_pair(const MGeo<float>& g, const MMassIO<float>& io,
-                                             float dt, float dt_h, const ItemBox box, int lane,
-                                             CSum& ms, CSum& es) {
-  using T = float;
+/* The y-first mass/energy march of an interior item, two columns per lane
+ * (mass_yx_box's interior form, column by column). */
+template <typename T, bool P2, bool UNI, typename Acc>
+__device__ __forceinline__ void mass_yx_pair(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                             T dt_h, const ItemBox box, int lane, Acc& ms,
+                                             Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
   if (y0 >= y1) return;
@@ -995,7 +1026,7 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<float>& g, const MMassIO
   const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
   const RowR<T, P2, UNI> rcdy(g.cdy, y0 - 4, ny, lane);
   auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
-  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_pair4(a, (long)row * sx + qb, x); };
+  auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_xpair(a, (long)row * sx + qb, x); };
 
   /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2: [row][column] */
   T wr[4][2], we[4][2], wv[3][2];
@@ -1080,7 +1111,7 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<float>& g, const MMassIO
       xm[c] = fx.mass;
       xe[c] = fx.energy;
     }
-    if (own) st_pair4(io.F_x, (long)r * sx + qb, xm);
+    if (own) st_xpair(io.F_x, (long)r * sx + qb, xm);
     const T xm_hi[2] = {pshift(xm, 0, 1), pshift(xm, 1, 1)};
     const T xe_hi[2] = {pshift(xe, 0, 1), pshift(xe, 1, 1)};
     if (own) {
@@ -1123,16 +1154,16 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGe
   FixSum fm{0, 0, 0, 0}, fe{0, 0, 0, 0};
   MarchQueue<true> mq(red.queue, ml.nitems);
   for (int item = mq.next(); item < mq.end; item = mq.next()) {
-    CSum ms{0.0, 0.0}, es{0.0, 0.0};
+    ItemSum<T> ms{0.0, 0.0}, es{0.0, 0.0};
     if constexpr (PAIR) {
       const ItemBox b = item_box(ml, g.nx, item);
       if (item_edge(ml, g.nx, g.ny, item)) {
         for (int sx0 = b.x0; sx0 < b.x1; sx0 += kMassW) {
           const ItemBox sb{sx0, min(sx0 + kMassW, b.x1), b.y0, b.y1};
-          mass_yx_box<T, P2, true, UNI>(g, io, dt, dt_h, sb, lane, ms, es);
+          mass_yx_box<T, P2, true, UNI, ItemSum<T>>(g, io, dt, dt_h, sb, lane, ms, es);
         }
       } else {
-        mass_yx_pair<P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
+        mass_yx_pair<T, P2, UNI>(g, io, dt, dt_h, b, lane, ms, es);
       }
     } else if (item_edge(ml, g.nx, g.ny, item)) {
       mass_yx_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, item, lane, ms, es);
@@ -1910,11 +1941,10 @@ __device__ __forceinline__ void lagrangian_item(const MGeo<T>& g, const MLagIO<T
 constexpr int kLagW = kWave - 6;       /* one column per lane */
 constexpr int kLagW2 = 2 * kWave - 8;  /* two */
 
-template <bool P2, bool UNI>
-__device__ __forceinline__ void lagrangian_pair(const MGeo<float>& g, const MLagIO<float>& io,
-                                                float dt, float dt_h, const ItemBox box,
-                                                int lane, double& local_min, int& nan_count) {
-  using T = float;
+template <typename T, bool P2, bool UNI>
+__device__ __forceinline__ void lagrangian_pair(const MGeo<T>& g, const MLagIO<T>& io, T dt,
+                                                T dt_h, const ItemBox box, int lane,
+                                                double& local_min, int& nan_count) {
   using P = T[2];
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   const int x0 = box.x0, x1 = box.x1, y0 = box.y0, y1 = box.y1;
@@ -1933,10 +1963,13 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<float>& g, const MLag
   }
   const RowS<T, UNI> edy(g.edy, y0 - 4, ny, lane), cdy(g.cdy, y0 - 4, ny, lane);
   const RowS<T, UNI> iedy(g.iedy, y0 - 4, ny, lane);
-  using Recip = typename std::conditional<UNI, RowConstRecip<T, P2>, RowGeo<T>>::type;
+  using Recip = typename std::conditional<
+      UNI, RowConstRecip<T, P2>,
+      typename std::conditional<std::is_same<T, float>::value, RowGeo<T>,
+                                RowRecip<T, P2>>::type>::type;
   const Recip rcdy(g.cdy, y0 - 4, ny, lane, true);
   auto ldc = [&](const T* a, int row, P& x) { ld_pair(a, (long)row * nx + qb, x); };
-  auto ldx = [&](const T* a, int row, P& x) { ld_pair4(a, (long)row * sx + qb, x); };
+  auto ldx = [&](const T* a, int row, P& x) { ld_xpair(a, (long)row * sx + qb, x); };
 
   /* y face k after pressure acceleration, from cell rows k (rk, ek) and
    * k-1 (rs, es) and mom_y at face k */
@@ -2058,8 +2091,8 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<float>& g, const MLag
     }
     const T uav_e[2] = {pshift(uav, 0, 1), pshift(uav, 1, 1)};
     if (own) {
-      st_pair4(io.mx1, (long)i * sx + qb, mx_av);
-      st_pair4(io.u0, (long)i * sx + qb, uav);
+      st_xpair(io.mx1, (long)i * sx + qb, mx_av);
+      st_xpair(io.u0, (long)i * sx + qb, uav);
       /* shock heating, CFL candidate and the copy of rho on row i */
       T es[2];
 #pragma unroll
@@ -2071,10 +2104,13 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<float>& g, const MLag
       st_pair(io.rho_old, (long)i * nx + qb, wr[0]);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        /* FP64 CFL candidate, squared */
-        const double cand = cell_timestep_sq(double(es[c]), double(qx[c]),
-                                             double(qy_prev[c]), double(wr[0][c]),
-                                             double(cdx_q[c]), double(cdy(i)));
+        /* FP64 CFL candidate (squared in the FP32-state form) */
+        const double cand =
+            std::is_same<T, float>::value
+                ? cell_timestep_sq(double(es[c]), double(qx[c]), double(qy_prev[c]),
+                                   double(wr[0][c]), double(cdx_q[c]), double(cdy(i)))
+                : cell_timestep(double(es[c]), double(qx[c]), double(qy_prev[c]),
+                                double(wr[0][c]), double(cdx_q[c]), double(cdy(i)));
         if (cand == cand)
           local_min = lesser(local_min, cand);
         else
@@ -2121,7 +2157,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(
        * sub-strips */
       const ItemBox b = item_box(ml, g.nx, item);
       if (!item_edge(ml, g.nx, g.ny, item)) {
-        lagrangian_pair<P2, UNI>(g, io, dt, dt_h, b, lane, local_min, nan_count);
+        lagrangian_pair<T, P2, UNI>(g, io, dt, dt_h, b, lane, local_min, nan_count);
       } else {
         for (int sx0 = b.x0; sx0 < b.x1; sx0 += kLagW) {
           const ItemBox sb{sx0, min(sx0 + kLagW, b.x1), b.y0, b.y1};

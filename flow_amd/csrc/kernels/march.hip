@@ -10,9 +10,9 @@ namespace flowhip {
  * units of H2 rows, chunks of k units, and nsmall one-unit chunks at the end
  * of each of the 8 XCD row bands (MarchList). Measured on MI355X at 8192^2,
  * uniform chunks (k = 1, nsmall = 0) of 24 / 20 / 16 rows (Lagrangian /
- * mass-energy / momentum; 24 / 16 / 20 in FP32, and 10 for the FP32
- * Lagrangian and mass/energy marches with two columns per lane) are
- * fastest: tapering from
+ * mass-energy / momentum; 24 / 16 / 20 in FP32; with two columns per lane
+ * 12 / 12 for the FP64 Lagrangian and mass/energy marches, 10 / 10 in
+ * FP32) are fastest: tapering from
  * tall chunks (48-80 rows) down to 12-24-row ones cut the re-read prologue
  * rows but made the Lagrangian and mass/energy marches 5-10% slower. An
  * explicit chunk_rows > 0 gives uniform chunks of that height. Every
@@ -48,6 +48,15 @@ struct Taper {
 #ifndef FLOWHIP_TAPER_LAG32P
 #define FLOWHIP_TAPER_LAG32P 10, 1, 0
 #endif
+/* the FP64 Lagrangian and mass/energy marches with two columns per lane
+ * (two waves per SIMD): 12 rows each (measured 8 to 24; 8 rows +2-5%,
+ * 24 +3%, 10 to 20 within 1%) */
+#ifndef FLOWHIP_TAPER_LAG64P
+#define FLOWHIP_TAPER_LAG64P 12, 1, 0
+#endif
+#ifndef FLOWHIP_TAPER_MASS64P
+#define FLOWHIP_TAPER_MASS64P 12, 1, 0
+#endif
 enum { kKLag = 0, kKMass = 1, kKMom = 2 };
 template <typename T>
 constexpr Taper taper_of(int kernel) {
@@ -69,19 +78,21 @@ inline MarchList march_items(int nx, int ny, int W, int chunk_rows, Taper t) {
 template <typename T, bool P2, bool UNI, bool PAIR = false>
 hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                                int chunk_rows, hipStream_t s) {
-  const MarchList ml = march_items(g.nx, g.ny, PAIR ? kMassW2 : kMassW, chunk_rows,
-                                   PAIR ? Taper{FLOWHIP_TAPER_MASS32P} : taper_of<T>(kKMass));
+  const MarchList ml = march_items(
+      g.nx, g.ny, PAIR ? kMassW2 : kMassW, chunk_rows,
+      PAIR ? (sizeof(T) == 8 ? Taper{FLOWHIP_TAPER_MASS64P} : Taper{FLOWHIP_TAPER_MASS32P})
+           : taper_of<T>(kKMass));
   if (ml.nitems < 1 || !r.queue || !r.totals) return hipErrorInvalidValue;
   if (tt % 2 == 0) {
     static int per_cu = 0;
     const int grid = march_grid(k_march_mass_xy<T, P2, UNI, PAIR>, per_cu, ml.nitems,
-                                PAIR ? kMassPairWaves : kMarchMaxWaves<T>);
+                                PAIR ? kMassPairWaves<T> : kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_mass_xy<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
                        g, io, (const double*)r.dtp, r, ml);
   } else {
     static int per_cu = 0;
     const int grid = march_grid(k_march_mass_yx<T, P2, UNI, PAIR>, per_cu, ml.nitems,
-                                PAIR ? kMassPairWaves : kMarchMaxWaves<T>);
+                                PAIR ? kMassPairWaves<T> : kMarchMaxWaves<T>);
     hipLaunchKernelGGL((k_march_mass_yx<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
                        g, io, (const double*)r.dtp, r, ml);
   }
@@ -92,35 +103,40 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
  * the cell and y-face arrays it reads or writes as column pairs start on
  * 8-byte boundaries (nx even, arrays 8-byte aligned; strips start at even
  * columns). */
-inline bool pairs_fit(const MGeo<float>& g, const MMassIO<float>& io) {
+template <typename T>
+inline bool pairs_fit(const MGeo<T>& g, const MMassIO<T>& io) {
   const unsigned long long bits =
       (unsigned long long)io.v0 | (unsigned long long)io.rho0 | (unsigned long long)io.e1 |
       (unsigned long long)io.rho | (unsigned long long)io.e | (unsigned long long)io.F_y;
-  return g.nx % 2 == 0 && (bits & 7ull) == 0;
+  return g.nx % 2 == 0 && (bits & (2 * sizeof(T) - 1)) == 0;
 }
 
 template <typename T, bool P2, bool UNI, bool PAIR = false>
 hipError_t launch_march_lagrangian_p(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                      int first_step, int chunk_rows, hipStream_t s) {
-  const MarchList ml = march_items(g.nx, g.ny, PAIR ? kLagW2 : kLagW, chunk_rows,
-                                   PAIR ? Taper{FLOWHIP_TAPER_LAG32P} : taper_of<T>(kKLag));
+  const MarchList ml = march_items(
+      g.nx, g.ny, PAIR ? kLagW2 : kLagW, chunk_rows,
+      PAIR ? (sizeof(T) == 8 ? Taper{FLOWHIP_TAPER_LAG64P} : Taper{FLOWHIP_TAPER_LAG32P})
+           : taper_of<T>(kKLag));
   if (ml.nitems < 1 || !r.queue) return hipErrorInvalidValue;
   static int per_cu = 0;
   const int grid = march_grid(k_march_lagrangian<T, P2, UNI, PAIR>, per_cu, ml.nitems,
-                              PAIR ? kLagPairWaves : kLagMaxWaves<T>);
+                              PAIR ? kLagPairWaves<T> : kLagMaxWaves<T>);
   hipLaunchKernelGGL((k_march_lagrangian<T, P2, UNI, PAIR>), dim3(grid), dim3(kMNT), 0, s,
                      g, io, (const double*)r.dtp, r, max_dt, first_step, ml);
   return hipGetLastError();
 }
 
-/* Whether an FP32 Lagrangian march can hold two columns per lane (as
- * pairs_fit, for the arrays it reads or writes in column pairs). */
-inline bool lag_pairs_fit(const MGeo<float>& g, const MLagIO<float>& io) {
+/* Whether a Lagrangian march can hold two columns per lane (as pairs_fit,
+ * for the arrays it reads or writes in column pairs: aligned to two
+ * values). */
+template <typename T>
+inline bool lag_pairs_fit(const MGeo<T>& g, const MLagIO<T>& io) {
   const unsigned long long bits =
       (unsigned long long)io.rho | (unsigned long long)io.e | (unsigned long long)io.mom_y |
       (unsigned long long)io.my1 | (unsigned long long)io.v0 | (unsigned long long)io.e1 |
       (unsigned long long)io.rho_old;
-  return g.nx % 2 == 0 && (bits & 7ull) == 0;
+  return g.nx % 2 == 0 && (bits & (2 * sizeof(T) - 1)) == 0;
 }
 
 /* Even-tt x momentum needs faces [x0-2, x1+1] and one lane below them. */
@@ -187,10 +203,8 @@ hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* 
 template <typename T>
 hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                              int chunk_rows, int pow2, int uniform, int pairs, hipStream_t s) {
-  if constexpr (sizeof(T) == 4) {
-    if (pairs && pow2 && uniform && pairs_fit(g, io))
-      return launch_march_mass_p<T, true, true, true>(g, io, tt, r, chunk_rows, s);
-  }
+  if (pairs && pow2 && uniform && pairs_fit(g, io))
+    return launch_march_mass_p<T, true, true, true>(g, io, tt, r, chunk_rows, s);
   if (pow2 && uniform) return launch_march_mass_p<T, true, true>(g, io, tt, r, chunk_rows, s);
   return pow2 ? launch_march_mass_p<T, true, false>(g, io, tt, r, chunk_rows, s)
               : launch_march_mass_p<T, false, false>(g, io, tt, r, chunk_rows, s);
@@ -202,10 +216,10 @@ template <typename T>
 hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                    int first_step, int chunk_rows, int pow2, int uniform,
                                    int pairs, hipStream_t s) {
+  if (pairs && pow2 && uniform && lag_pairs_fit(g, io))
+    return launch_march_lagrangian_p<T, true, true, true>(g, io, r, max_dt, first_step,
+                                                          chunk_rows, s);
   if constexpr (sizeof(T) == 4) {
-    if (pairs && pow2 && uniform && lag_pairs_fit(g, io))
-      return launch_march_lagrangian_p<T, true, true, true>(g, io, r, max_dt, first_step,
-                                                            chunk_rows, s);
     if (pow2 && uniform)
       return launch_march_lagrangian_p<T, true, true>(g, io, r, max_dt, first_step,
                                                       chunk_rows, s);
@@ -253,7 +267,7 @@ hipError_t fk_march_lagrangian(FkMesh m, const double* rho, const double* e,
                                int first_step, int chunk_rows, hipStream_t s) {
   MLagIO<double> io{rho, e, mom_x, mom_y, mx1, my1, u0, v0, e1, rho_old, r.cfl};
   return launch_march_lagrangian<double>(mgeo64(m), io, r, max_dt, first_step,
-                                         chunk_rows, m.pow2, m.uniform, 0, s);
+                                         chunk_rows, m.pow2, m.uniform, m.pairs, s);
 }
 
 hipError_t fk_march_momentum(FkMesh m, int tt, const double* u0, const double* v0,
@@ -290,8 +304,8 @@ hipError_t fk_march_mass_energy(FkMesh m, int tt, const double* u0,
                                 double* F_x, double* F_y, FkReduce r,
                                 int chunk_rows, hipStream_t s) {
   MMassIO<double> io{u0, v0, rho0, e1, rho, e, F_x, F_y};
-  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, m.uniform, 0,
-                                   s);
+  return launch_march_mass<double>(mgeo64(m), io, tt, r, chunk_rows, m.pow2, m.uniform,
+                                   m.pairs, s);
 }
 
 static MGeo<float> mgeo32(FkMesh32 m) {

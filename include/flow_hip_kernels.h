@@ -34,6 +34,11 @@ typedef struct FkMesh {
    * throughout (the mass/energy and momentum marches then read the row
    * geometry once per work item instead of once per row), else 0 */
   int uniform;
+  /* 1 when the Lagrangian and mass/energy marches may hold two columns
+   * per lane on uniform power-of-two meshes (each does when nx is even and
+   * the cell and y-face arrays it reads and writes are 16-byte aligned),
+   * else 0 */
+  int pairs;
 } FkMesh;
 
 /* Geometry of the FP32-state precision form: the same arrays rounded to
@@ -44,11 +49,7 @@ typedef struct FkMesh32 {
   const float *inv_edgedx, *inv_edgedy;
   int pow2;
   int uniform; /* as in FkMesh */
-  /* 1 when the Lagrangian and mass/energy marches may hold two columns
-   * per lane on uniform power-of-two meshes (each does when nx is even and
-   * the cell and y-face arrays it reads and writes are 8-byte aligned),
-   * else 0 */
-  int pairs;
+  int pairs;   /* as in FkMesh (8-byte alignment) */
 } FkMesh32;
 
 /* Device scratch for the single-pass grid reductions. */

@@ -5,9 +5,11 @@ quotients by spacings with products by exact reciprocals, and when the
 spacings are also uniform the mass/energy and momentum marches read the row
 geometry once per work item. FLOWHIP_UNIFORM=0 and FLOWHIP_POW2=0 (read
 when a mesh is created) switch those forms off. Each problem here runs in
-all three forms, FP64 and FP32 state, and every array, the dt sequence and
-the totals must agree bit for bit; the FP64 default form is also checked
-against the oracle.
+all three forms, FP64 and FP32 state, and every array and the dt sequence
+must agree bit for bit; the FP64 default form is also checked against the
+oracle. The totals must agree to 1e-12: on uniform power-of-two meshes the
+Lagrangian and mass/energy marches hold two columns per lane, whose
+per-item partial sums round differently from the one-column form's.
 """
 import numpy as np
 import pytest
@@ -71,8 +73,8 @@ def test_geometry_forms_agree(monkeypatch, name, n, steps, dt, fp32):
                                          a.view(np.uint8)))
             assert ndiff == 0, (form, k, ndiff)
         np.testing.assert_array_equal(got["dt"], base["dt"])
-        np.testing.assert_array_equal(got["mass"], base["mass"])
-        np.testing.assert_array_equal(got["energy"], base["energy"])
+        np.testing.assert_allclose(got["mass"], base["mass"], rtol=1e-12)
+        np.testing.assert_allclose(got["energy"], base["energy"], rtol=1e-12)
     if not fp32:
         ref = harness.oracle_run(None, n, steps, dt=dt, init=_init(name, n))
         for k, (rel, ndiff) in harness.compare_state(base["arrays"], ref["arrays"],

@@ -1,21 +1,22 @@
-"""The FP32 mass/energy and Lagrangian marches with two columns per lane
-give the fields of the one-column marches bit for bit.
+"""The Lagrangian and mass/energy marches with two columns per lane give the
+fields of the one-column marches bit for bit, FP64 and FP32 state.
 
-On uniform power-of-two meshes with an even padded width the FP32-state
-form runs the interior items of those two marches two columns per lane
-(strips of 124 and 120 columns); edge items go through the one-column form
-in sub-strips. FLOWHIP_FP32_PAIRS=0 (read when a mesh is created) keeps one
+On uniform power-of-two meshes with an even padded width the march
+schedule runs the interior items of those two marches two columns per lane
+(strips of 120 and 124 columns); edge items go through the one-column form
+in sub-strips. FLOWHIP_PAIRS=0 (read when a mesh is created) keeps one
 column per lane. Each case runs both ways over both sweep orders; every
 array and the dt sequence must agree bit for bit (the CFL minimum does not
 depend on the order the candidates are taken in), and the mass and energy
-totals to 1e-12 (each lane's compensated sum then adds two columns a row,
-so the rounding of the per-item sums differs).
+totals to 1e-12: each lane's sums then take two columns a row over other
+chunk heights, so the rounding of the per-item partial sums differs.
 """
 import numpy as np
 import pytest
 
 import flow_amd
 from flow_amd import problems
+from tests import harness
 
 pytestmark = pytest.mark.gpu
 
@@ -35,11 +36,13 @@ def _init(n):
                                    (0.5, 50.0, 0.1, 0.6, 0.2, 0.15)])
 
 
-def _run(monkeypatch, pairs, n, steps, dt):
-    monkeypatch.delenv("FLOWHIP_FP32_PAIRS", raising=False)
+def _run(monkeypatch, pairs, n, steps, dt, fp32):
+    monkeypatch.delenv("FLOWHIP_PAIRS", raising=False)
     if not pairs:
-        monkeypatch.setenv("FLOWHIP_FP32_PAIRS", "0")
+        monkeypatch.setenv("FLOWHIP_PAIRS", "0")
     rho, e = _init(n)
+    if not fp32:
+        return harness.gpu_run(None, n, steps, flow_amd.MARCH, dt=dt, init=(rho, e))
     m = flow_amd.Mesh(n[0], n[1], dt=dt)
     st = flow_amd.DeviceState32(m)
     st.upload("rho", rho)
@@ -52,11 +55,12 @@ def _run(monkeypatch, pairs, n, steps, dt):
     return out
 
 
+@pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("n,steps,dt", CASES)
-def test_fp32_pairs_equal_one_column(monkeypatch, n, steps, dt):
-    print(f"case fp32 pairs n={n} steps={steps}", flush=True)
-    one = _run(monkeypatch, False, n, steps, dt)
-    two = _run(monkeypatch, True, n, steps, dt)
+def test_pairs_equal_one_column(monkeypatch, n, steps, dt, fp32):
+    print(f"case pairs n={n} steps={steps} fp32={fp32}", flush=True)
+    one = _run(monkeypatch, False, n, steps, dt, fp32)
+    two = _run(monkeypatch, True, n, steps, dt, fp32)
     assert one["steps"] == two["steps"] == steps
     for k, a in one["arrays"].items():
         ndiff = int(np.count_nonzero(two["arrays"][k].view(np.uint8) != a.view(np.uint8)))
