@@ -726,31 +726,40 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<T>& g, const MMassIO<T>&
   auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
   auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_xpair(a, (long)row * sx + qb, x); };
 
-  /* windows: [row][column], row 0 oldest */
-  T wr[4][2], we[4][2], vy[3][2], r0[3][2];
-  T fy_prev[2] = {T(0), T(0)}, efy_prev[2] = {T(0), T(0)};
+  /* Row rings of four slots, row k in slot (k - rb) & 3: rho0 and v0 rows
+   * r-2 .. r+1, the x-updated rho and e rows r-3 .. r; e1 and u0 rows r,
+   * r+1 in two slots, the y fluxes of faces r-2, r-1 likewise. The row loop
+   * is unrolled by four with the slot of each row fixed at compile time,
+   * so no window is copied from row to row. */
+  T r0[4][2], vy[4][2], wr[4][2], we[4][2], eb[2][2], ub[2][2];
+  T fyb[2][2], efyb[2][2];
   const int rb = y0 - 2;
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     wr[k][0] = wr[k][1] = we[k][0] = we[k][1] = T(0);
-  ldc(io.v0, rb - 2, vy[0]);
-  ldc(io.v0, rb - 1, vy[1]);
-  ldc(io.rho0, rb - 2, r0[0]);
-  ldc(io.rho0, rb - 1, r0[1]);
-  T a_r[2], a_e[2], a_u[2], a_v[2];
-  ldc(io.rho0, rb, a_r);
-  ldc(io.e1, rb, a_e);
-  ldx(io.u0, rb, a_u);
-  ldc(io.v0, rb, a_v);
-  for (int r = rb; r <= y1 + 1; ++r) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    fyb[k][0] = fyb[k][1] = efyb[k][0] = efyb[k][1] = T(0);
+  ldc(io.v0, rb - 2, vy[2]);
+  ldc(io.v0, rb - 1, vy[3]);
+  ldc(io.rho0, rb - 2, r0[2]);
+  ldc(io.rho0, rb - 1, r0[3]);
+  ldc(io.rho0, rb, r0[0]);
+  ldc(io.e1, rb, eb[0]);
+  ldx(io.u0, rb, ub[0]);
+  ldc(io.v0, rb, vy[0]);
+  auto row = [&](auto phase, int r) {
+    constexpr int P = decltype(phase)::value;
+    constexpr int S0 = P, S1 = (P + 1) & 3, S2 = (P + 2) & 3, S3 = (P + 3) & 3;
+    constexpr int A = P & 1, B = (P + 1) & 1;
     /* the next row (an interior item's rows all lie inside the mesh) */
-    T n_r[2], n_e[2], n_u[2], n_v[2];
-    ldc(io.rho0, r + 1, n_r);
-    ldc(io.e1, r + 1, n_e);
-    ldx(io.u0, r + 1, n_u);
-    ldc(io.v0, r + 1, n_v);
-    copy_pair(vy[2], a_v);
-    copy_pair(r0[2], a_r);
+    ldc(io.rho0, r + 1, r0[S1]);
+    ldc(io.e1, r + 1, eb[B]);
+    ldx(io.u0, r + 1, ub[B]);
+    ldc(io.v0, r + 1, vy[S1]);
+    const T(&a_r)[2] = r0[S0];
+    const T(&a_e)[2] = eb[A];
+    const T(&a_u)[2] = ub[A];
 
     /* x fluxes and the x update of row r */
     T fm[2], fe[2];
@@ -768,44 +777,36 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<T>& g, const MMassIO<T>&
       fe[c] = fl.energy;
     }
     if (own && r >= y0 && r < y1) st_xpair(io.F_x, (long)r * sx + qb, fm);
-    T xr[2], xe[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const CellState<T> st =
           sweep_update<T, P2>(a_r[c], a_r[c], a_e[c], fm[c], pshift(fm, c, 1), fe[c],
                               pshift(fe, c, 1), edy(r), edy(r + 1), cdx_q[c], cdy(r), dt_h,
                               true, rcdx_q[c] * rcdy(r));
-      xr[c] = st.rho;
-      xe[c] = st.e;
+      wr[S0][c] = st.rho;
+      we[S0][c] = st.e;
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      copy_pair(wr[k], wr[k + 1]);
-      copy_pair(we[k], we[k + 1]);
-    }
-    copy_pair(wr[3], xr);
-    copy_pair(we[3], xe);
-    /* y flux at face r-1 from rows r-3 .. r, and the y update of row r-2 */
+    /* y flux at face r-1 from x-updated rows r-3 .. r, and the y update of
+     * row r-2 */
     const int gf = r - 1;
     if (gf >= y0 && gf <= y1) {
-      T fy[2], efy[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
-            vy[0][c], vy[1][c], vy[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c],
-            we[0][c], we[1][c], we[2][c], we[3][c], iedy(gf), cdy(gf - 1), cdy(gf),
+            vy[S2][c], vy[S3][c], vy[S0][c], wr[S1][c], wr[S2][c], wr[S3][c], wr[S0][c],
+            we[S1][c], we[S2][c], we[S3][c], we[S0][c], iedy(gf), cdy(gf - 1), cdy(gf),
             edx_q[c], dt, dt_h, rcdy(gf));
-        fy[c] = fl.mass;
-        efy[c] = fl.energy;
+        fyb[A][c] = fl.mass;
+        efyb[A][c] = fl.energy;
       }
-      if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fy);
+      if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fyb[A]);
       const int i = r - 2;
       if (i >= y0 && own) {
         T nr[2], ne[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
           const CellState<T> st = sweep_update<T, P2>(
-              wr[1][c], r0[0][c], we[1][c], fy_prev[c], fy[c], efy_prev[c], efy[c],
+              wr[S2][c], r0[S2][c], we[S2][c], fyb[B][c], fyb[A][c], efyb[B][c], efyb[A][c],
               edx_q[c], edx_q1[c], cdx_q[c], cdy(i), dt_h, false, rcdx_q[c] * rcdy(i));
           nr[c] = st.rho;
           ne[c] = st.e;
@@ -818,18 +819,22 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<T>& g, const MMassIO<T>&
           es.add(double(nr[c]) * double(ne[c]));
         }
       }
-      copy_pair(fy_prev, fy);
-      copy_pair(efy_prev, efy);
     }
-    copy_pair(vy[0], vy[1]);
-    copy_pair(vy[1], vy[2]);
-    copy_pair(r0[0], r0[1]);
-    copy_pair(r0[1], r0[2]);
-    copy_pair(a_r, n_r);
-    copy_pair(a_e, n_e);
-    copy_pair(a_u, n_u);
-    copy_pair(a_v, n_v);
+  };
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
+  using P2c = std::integral_constant<int, 2>;
+  using P3 = std::integral_constant<int, 3>;
+  int r = rb;
+  for (; r + 3 <= y1 + 1; r += 4) {
+    row(P0{}, r);
+    row(P1{}, r + 1);
+    row(P2c{}, r + 2);
+    row(P3{}, r + 3);
   }
+  if (r <= y1 + 1) row(P0{}, r);
+  if (r + 1 <= y1 + 1) row(P1{}, r + 1);
+  if (r + 2 <= y1 + 1) row(P2c{}, r + 2);
 }
 
 /* PAIR: interior items two columns per lane (FP32, strips of kMassW2
@@ -1028,74 +1033,67 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<T>& g, const MMassIO<T>&
   auto ldc = [&](const T* a, int row, T(&x)[2]) { ld_pair(a, (long)row * nx + qb, x); };
   auto ldx = [&](const T* a, int row, T(&x)[2]) { ld_xpair(a, (long)row * sx + qb, x); };
 
-  /* rho0, e1 at rows r-1 .. r+2 and v0 at faces r .. r+2: [row][column] */
-  T wr[4][2], we[4][2], wv[3][2];
+  /* Rings of four slots: rho0 and e1 row k in slot (k - y0 + 2) & 3 (rows
+   * r-1 .. r+2 in use at row r), v0 face k in slot (k - y0) & 3 (faces
+   * r .. r+3); u0 rows and the y fluxes at faces r, r+1 in two slots. The
+   * row loop is unrolled by four with every slot fixed at compile time, so
+   * no window is copied from row to row. */
+  T wr[4][2], we[4][2], wv[4][2], ub[2][2], fyb[2][2], efyb[2][2];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     ldc(io.rho0, y0 - 2 + k, wr[k]);
     ldc(io.e1, y0 - 2 + k, we[k]);
   }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) ldc(io.v0, y0 - 1 + k, wv[k]);
+  ldc(io.v0, y0 - 1, wv[3]);
+  ldc(io.v0, y0, wv[0]);
+  ldc(io.v0, y0 + 1, wv[1]);
   /* y flux at face y0 (rows y0-2 .. y0+1) */
-  T fpm[2], fpe[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
-        wv[0][c], wv[1][c], wv[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c], we[0][c],
+        wv[3][c], wv[0][c], wv[1][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c], we[0][c],
         we[1][c], we[2][c], we[3][c], iedy(y0), cdy(y0 - 1), cdy(y0), edx_q[c], dt, dt_h,
         rcdy(y0));
-    fpm[c] = fl.mass;
-    fpe[c] = fl.energy;
+    fyb[0][c] = fl.mass;
+    efyb[0][c] = fl.energy;
   }
-  if (own) st_pair(io.F_y, (long)y0 * nx + qb, fpm);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    copy_pair(wr[k], wr[k + 1]);
-    copy_pair(we[k], we[k + 1]);
-  }
-  copy_pair(wv[0], wv[1]);
-  copy_pair(wv[1], wv[2]);
-  T n_r[2], n_e[2], n_v[2], a_u[2], a_r0[2];
-  ldc(io.rho0, y0 + 2, n_r);
-  ldc(io.e1, y0 + 2, n_e);
-  ldc(io.v0, y0 + 2, n_v);
-  ldx(io.u0, y0, a_u);
-  ldc(io.rho0, y0, a_r0);
-  for (int r = y0; r < y1; ++r) {
-    copy_pair(wr[3], n_r);
-    copy_pair(we[3], n_e);
-    copy_pair(wv[2], n_v);
-    T c_u[2], c_r0[2];
-    copy_pair(c_u, a_u);
-    copy_pair(c_r0, a_r0);
+  if (own) st_pair(io.F_y, (long)y0 * nx + qb, fyb[0]);
+  ldc(io.rho0, y0 + 2, wr[0]);
+  ldc(io.e1, y0 + 2, we[0]);
+  ldc(io.v0, y0 + 2, wv[2]);
+  ldx(io.u0, y0, ub[0]);
+  auto row = [&](auto phase, int r) {
+    constexpr int P = decltype(phase)::value;
+    /* rho0, e1 rows r-1, r, r+1, r+2; v0 faces r .. r+3 */
+    constexpr int R1 = (P + 1) & 3, R2 = (P + 2) & 3, R3 = (P + 3) & 3, R0 = P;
+    constexpr int V0 = P, V1 = (P + 1) & 3, V2 = (P + 2) & 3, V3 = (P + 3) & 3;
+    constexpr int A = P & 1, B = (P + 1) & 1;
     /* the next row's inputs (an interior item's rows all lie inside) */
-    ldc(io.rho0, r + 3, n_r);
-    ldc(io.e1, r + 3, n_e);
-    ldc(io.v0, r + 3, n_v);
-    ldx(io.u0, r + 1, a_u);
-    ldc(io.rho0, r + 1, a_r0);
+    ldc(io.v0, r + 3, wv[V3]);
+    ldx(io.u0, r + 1, ub[B]);
+    const T(&c_u)[2] = ub[A];
     /* y flux at face r+1 (rows r-1 .. r+2) and the y update of row r */
     const int gf = r + 1;
-    T fm[2], fe[2], yr[2], ye[2];
+    T yr[2], ye[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const FaceFlux<T> fl = mass_energy_flux<T, true, P2>(
-          wv[0][c], wv[1][c], wv[2][c], wr[0][c], wr[1][c], wr[2][c], wr[3][c], we[0][c],
-          we[1][c], we[2][c], we[3][c], iedy(gf), cdy(gf - 1), cdy(gf), edx_q[c], dt, dt_h,
-          rcdy(gf));
-      fm[c] = fl.mass;
-      fe[c] = fl.energy;
+          wv[V0][c], wv[V1][c], wv[V2][c], wr[R1][c], wr[R2][c], wr[R3][c], wr[R0][c],
+          we[R1][c], we[R2][c], we[R3][c], we[R0][c], iedy(gf), cdy(gf - 1), cdy(gf),
+          edx_q[c], dt, dt_h, rcdy(gf));
+      fyb[B][c] = fl.mass;
+      efyb[B][c] = fl.energy;
       const CellState<T> ys =
-          sweep_update<T, P2>(wr[1][c], wr[1][c], we[1][c], fpm[c], fl.mass, fpe[c],
-                              fl.energy, edx_q[c], edx_q1[c], cdx_q[c], cdy(r), dt_h, true,
-                              rcdx_q[c] * rcdy(r));
+          sweep_update<T, P2>(wr[R2][c], wr[R2][c], we[R2][c], fyb[A][c], fl.mass,
+                              efyb[A][c], fl.energy, edx_q[c], edx_q1[c], cdx_q[c], cdy(r),
+                              dt_h, true, rcdx_q[c] * rcdy(r));
       yr[c] = ys.rho;
       ye[c] = ys.e;
     }
-    if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fm);
-    copy_pair(fpm, fm);
-    copy_pair(fpe, fe);
+    /* row r+3 into the slot of row r-1 */
+    ldc(io.rho0, r + 3, wr[R1]);
+    ldc(io.e1, r + 3, we[R1]);
+    if (own && gf < y1) st_pair(io.F_y, (long)gf * nx + qb, fyb[B]);
     /* x fluxes of the y-updated row, then the x update */
     T xm[2], xe[2];
 #pragma unroll
@@ -1119,7 +1117,7 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<T>& g, const MMassIO<T>&
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const CellState<T> st = sweep_update<T, P2>(
-            yr[c], c_r0[c], ye[c], xm[c], xm_hi[c], xe[c], xe_hi[c], edy(r), edy(r + 1),
+            yr[c], wr[R2][c], ye[c], xm[c], xm_hi[c], xe[c], xe_hi[c], edy(r), edy(r + 1),
             cdx_q[c], cdy(r), dt_h, false, rcdx_q[c] * rcdy(r));
         nr[c] = st.rho;
         ne[c] = st.e;
@@ -1132,14 +1130,21 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<T>& g, const MMassIO<T>&
         es.add(double(nr[c]) * double(ne[c]));
       }
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      copy_pair(wr[k], wr[k + 1]);
-      copy_pair(we[k], we[k + 1]);
-    }
-    copy_pair(wv[0], wv[1]);
-    copy_pair(wv[1], wv[2]);
+  };
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
+  using P2c = std::integral_constant<int, 2>;
+  using P3 = std::integral_constant<int, 3>;
+  int r = y0;
+  for (; r + 3 < y1; r += 4) {
+    row(P0{}, r);
+    row(P1{}, r + 1);
+    row(P2c{}, r + 2);
+    row(P3{}, r + 3);
   }
+  if (r < y1) row(P0{}, r);
+  if (r + 1 < y1) row(P1{}, r + 1);
+  if (r + 2 < y1) row(P2c{}, r + 2);
 }
 
 template <typename T, bool P2, bool UNI = false, bool PAIR = false>
@@ -2021,66 +2026,74 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<T>& g, const MLagIO<T
       st_pair(io.v0, (long)y0 * nx + qb, vav_prev);
     }
   }
-  /* windows entering row y0: v at faces y0 .. y0+2; mom, rho edge at faces
-   * y0+1, y0+2 */
-  T wv[4][2], wm[3][2], wx[3][2];
+  /* Rings of four slots, row or face k in slot (k - y0) & 3: rho and e rows
+   * i .. i+3, v at y faces i .. i+3, mom and rho edge at faces i+1 .. i+3.
+   * Two slots (k - y0) & 1: Qyy of rows i, i+1, v after the viscous step at
+   * faces i, i+1, mom_y at faces i+3, i+4, mom_x at rows i, i+1, and the
+   * prefetched rho and e rows i+4, i+5 (copied into the row ring when row i
+   * leaves it). The row loop is unrolled by four with every slot fixed at
+   * compile time. */
+  T wv[4][2], wm[4][2], wx[4][2], qyb[2][2], vavb[2][2], myb[2][2], mxb[2][2];
+  T pfr[2][2], pfe[2][2];
   copy_pair(wv[0], vp[2]);
   copy_pair(wv[1], vp[3]);
   copy_pair(wv[2], vp[4]);
-  copy_pair(wm[0], pm[3]);
-  copy_pair(wm[1], pm[4]);
-  copy_pair(wx[0], pr[3]);
-  copy_pair(wx[1], pr[4]);
+  copy_pair(wm[1], pm[3]);
+  copy_pair(wm[2], pm[4]);
+  copy_pair(wx[1], pr[3]);
+  copy_pair(wx[2], pr[4]);
+  copy_pair(qyb[0], qy_prev);
+  copy_pair(vavb[0], vav_prev);
   ldc(io.rho, y0 + 3, wr[3]);
   ldc(io.e, y0 + 3, we[3]);
-  T a_my[2], a_mx[2], a_r[2], a_e[2];
-  ldc(io.mom_y, y0 + 3, a_my);
-  ldx(io.mom_x, y0, a_mx);
-  ldc(io.rho, y0 + 4, a_r);
-  ldc(io.e, y0 + 4, a_e);
+  ldc(io.mom_y, y0 + 3, myb[1]);
+  ldx(io.mom_x, y0, mxb[0]);
+  ldc(io.rho, y0 + 4, pfr[0]);
+  ldc(io.e, y0 + 4, pfe[0]);
 
-  for (int i = y0; i < y1; ++i) {
-    T c_my[2], c_mx[2], n_r[2], n_e[2];
-    copy_pair(c_my, a_my);
-    copy_pair(c_mx, a_mx);
-    copy_pair(n_r, a_r);
-    copy_pair(n_e, a_e);
+  auto row = [&](auto phase, int i) {
+    constexpr int P = decltype(phase)::value;
+    constexpr int S0 = P, S1 = (P + 1) & 3, S2 = (P + 2) & 3, S3 = (P + 3) & 3;
+    constexpr int A = P & 1, B = (P + 1) & 1;
     /* the next rows (an interior item's rows all lie inside the mesh) */
-    ldc(io.mom_y, i + 4, a_my);
-    ldx(io.mom_x, i + 1, a_mx);
-    ldc(io.rho, i + 5, a_r);
-    ldc(io.e, i + 5, a_e);
+    ldc(io.mom_y, i + 4, myb[A]);
+    ldx(io.mom_x, i + 1, mxb[B]);
+    ldc(io.rho, i + 5, pfr[B]);
+    ldc(io.e, i + 5, pfe[B]);
     /* y face i+3 after pressure acceleration */
-    ypa(i + 3, wr[3], we[3], wr[2], we[2], c_my, wm[2], wx[2], wv[3]);
+    ypa(i + 3, wr[S3], we[S3], wr[S2], we[S2], myb[B], wm[S3], wx[S3], wv[S3]);
     /* Qyy at cell row i+1, then y face i+1 after the viscous acceleration */
-    T qy[2], my_av[2], vav[2];
+    T my_av[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      qy[c] = viscous_stress(wr[1][c], wv[0][c], wv[1][c], wv[2][c], wv[3][c]);
-      my_av[c] = accelerate<T, P2>(wm[0][c], dt, qy[c], qy_prev[c], cdy(i + 1), rcdy(i + 1));
-      vav[c] = velocity(my_av[c], wx[0][c]);
+      qyb[B][c] = viscous_stress(wr[S1][c], wv[S0][c], wv[S1][c], wv[S2][c], wv[S3][c]);
+      my_av[c] = accelerate<T, P2>(wm[S1][c], dt, qyb[B][c], qyb[A][c], cdy(i + 1),
+                                   rcdy(i + 1));
+      vavb[B][c] = velocity(my_av[c], wx[S1][c]);
     }
     if (own && i + 1 < y1) {
       st_pair(io.my1, (long)(i + 1) * nx + qb, my_av);
-      st_pair(io.v0, (long)(i + 1) * nx + qb, vav);
+      st_pair(io.v0, (long)(i + 1) * nx + qb, vavb[B]);
     }
 
     /* along x on row i */
+    const T(&ri)[2] = wr[S0];
+    const T(&ei)[2] = we[S0];
     T Pr[2], mx_pa[2], rex[2], u_pa[2];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) Pr[c] = eos_pressure(wr[0][c], we[0][c]);
+    for (int c = 0; c < 2; ++c) Pr[c] = eos_pressure(ri[c], ei[c]);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      mx_pa[c] = accelerate<T, P2>(c_mx[c], dt, Pr[c], pshift(Pr, c, -1), edx_q[c],
+      mx_pa[c] = accelerate<T, P2>(mxb[A][c], dt, Pr[c], pshift(Pr, c, -1), edx_q[c],
                                    iedx_q[c]);
-      rex[c] = rho_edge_x<T, P2>(wr[0][c], pshift(wr[0], c, -1), cdx_q[c], cdx_qm[c], cdy(i),
+      rex[c] = rho_edge_x<T, P2>(ri[c], pshift(ri, c, -1), cdx_q[c], cdx_qm[c], cdy(i),
                                  edx_q[c], T(0.5) * iedx_q[c] * rcdy(i));
       u_pa[c] = velocity(mx_pa[c], rex[c]);
     }
     T qx[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
-      qx[c] = viscous_stress(wr[0][c], pshift(u_pa, c, -1), u_pa[c], pshift(u_pa, c, 1),
+      qx[c] = viscous_stress(ri[c], pshift(u_pa, c, -1), u_pa[c], pshift(u_pa, c, 1),
                              pshift(u_pa, c, 2));
     T mx_av[2], uav[2];
 #pragma unroll
@@ -2097,20 +2110,20 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<T>& g, const MLagIO<T
       T es[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
-        es[c] = shock_heating<T, P2>(we[0][c], Pr[c], wr[0][c], qx[c], qy_prev[c], uav[c],
-                                     uav_e[c], vav_prev[c], vav[c], cdx_q[c], cdy(i), dt_h,
-                                     rcdx_q[c], rcdy(i));
+        es[c] = shock_heating<T, P2>(ei[c], Pr[c], ri[c], qx[c], qyb[A][c], uav[c],
+                                     uav_e[c], vavb[A][c], vavb[B][c], cdx_q[c], cdy(i),
+                                     dt_h, rcdx_q[c], rcdy(i));
       st_pair(io.e1, (long)i * nx + qb, es);
-      st_pair(io.rho_old, (long)i * nx + qb, wr[0]);
+      st_pair(io.rho_old, (long)i * nx + qb, ri);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         /* FP64 CFL candidate (squared in the FP32-state form) */
         const double cand =
             std::is_same<T, float>::value
-                ? cell_timestep_sq(double(es[c]), double(qx[c]), double(qy_prev[c]),
-                                   double(wr[0][c]), double(cdx_q[c]), double(cdy(i)))
-                : cell_timestep(double(es[c]), double(qx[c]), double(qy_prev[c]),
-                                double(wr[0][c]), double(cdx_q[c]), double(cdy(i)));
+                ? cell_timestep_sq(double(es[c]), double(qx[c]), double(qyb[A][c]),
+                                   double(ri[c]), double(cdx_q[c]), double(cdy(i)))
+                : cell_timestep(double(es[c]), double(qx[c]), double(qyb[A][c]),
+                                double(ri[c]), double(cdx_q[c]), double(cdy(i)));
         if (cand == cand)
           local_min = lesser(local_min, cand);
         else
@@ -2118,27 +2131,35 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<T>& g, const MLagIO<T
         if (io.cfl) io.cfl[(long)i * nx + qb + c] = cand;
       }
     }
-
-    /* advance the windows */
-    copy_pair(qy_prev, qy);
-    copy_pair(vav_prev, vav);
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      copy_pair(wv[s], wv[s + 1]);
-      copy_pair(wr[s], wr[s + 1]);
-      copy_pair(we[s], we[s + 1]);
-    }
-    copy_pair(wr[3], n_r);
-    copy_pair(we[3], n_e);
-    copy_pair(wm[0], wm[1]);
-    copy_pair(wm[1], wm[2]);
-    copy_pair(wx[0], wx[1]);
-    copy_pair(wx[1], wx[2]);
+    /* row i+4 takes row i's slot */
+    copy_pair(wr[S0], pfr[A]);
+    copy_pair(we[S0], pfe[A]);
+  };
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
+  using P2c = std::integral_constant<int, 2>;
+  using P3 = std::integral_constant<int, 3>;
+  int i = y0;
+  for (; i + 3 < y1; i += 4) {
+    row(P0{}, i);
+    row(P1{}, i + 1);
+    row(P2c{}, i + 2);
+    row(P3{}, i + 3);
   }
+  if (i < y1) row(P0{}, i);
+  if (i + 1 < y1) row(P1{}, i + 1);
+  if (i + 2 < y1) row(P2c{}, i + 2);
 }
 
+/* Waves per SIMD the register allocation of a Lagrangian march must allow:
+ * the FP32 two-column form runs three (its loop, unrolled, would
+ * otherwise take 170 VGPRs and leave room for two). */
+template <typename T, bool PAIR>
+constexpr int kLagMinWaves = (PAIR && sizeof(T) == 4) ? kLagPairWaves<T> : 1;
+
 template <typename T, bool P2, bool UNI = false, bool PAIR = false>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
+__global__ __launch_bounds__(kMNT) __attribute__((amdgpu_waves_per_eu(kLagMinWaves<T, PAIR>)))
+void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
                                                            const double* dtp,
                                                            FkReduce red, double max_dt,
                                                            int first_step, MarchList ml) {
