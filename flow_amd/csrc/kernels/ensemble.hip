@@ -94,8 +94,13 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     /* FP32 state: squared candidates (cell_timestep_sq) */
     double local_min = std::is_same<T, float>::value ? in.max_dt * in.max_dt : in.max_dt;
     int nan_count = 0;
-    lagrangian_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane,
-                    local_min, nan_count);
+    const MGeo<T> g = ens_geo<T>(in);
+    const MarchList ml = ens_list(in, H);
+    /* interior items take the straight-line form, as on a single mesh */
+    if (!item_edge(ml, g.nx, g.ny, wi.y))
+      lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane, local_min, nan_count);
+    else
+      lagrangian_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, local_min, nan_count);
     local_min = wave_min(local_min);
     nan_count = wave_sum_int(nan_count);
     if (lane == 0) {
@@ -121,10 +126,20 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
                         arr<T>(in, kFx),  arr<T>(in, kFy)};
     const T dt = T(in.dtp[0]), dt_h = T(in.dtp[1]);
     CSum ms{0.0, 0.0}, es{0.0, 0.0};
-    if (tt % 2 == 0)
-      mass_xy_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
-    else
-      mass_yx_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane, ms, es);
+    const MGeo<T> g = ens_geo<T>(in);
+    const MarchList ml = ens_list(in, H);
+    const bool edge = item_edge(ml, g.nx, g.ny, wi.y);
+    if (tt % 2 == 0) {
+      if (edge)
+        mass_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+      else
+        mass_xy_item<T, P2, false>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+    } else {
+      if (edge)
+        mass_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+      else
+        mass_yx_item<T, P2, false>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+    }
     ms = wave_csum(ms);
     es = wave_csum(es);
     if (lane == 0) {
@@ -193,10 +208,20 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
                         arr<T>(in, kFy),  arr<T>(in, kRho), arr<T>(in, kQxx),
                         arr<T>(in, kQyy), arr<T>(in, kMomX), arr<T>(in, kMomY),
                         arr<T>(in, kU),   arr<T>(in, kV)};
-    if (tt % 2)
-      momxy_xy_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane);
-    else
-      momxy_yx_item<T, P2>(ens_geo<T>(in), io, dt, dt_h, ens_list(in, H), wi.y, lane);
+    const MGeo<T> g = ens_geo<T>(in);
+    const MarchList ml = ens_list(in, H);
+    const bool edge = item_edge(ml, g.nx, g.ny, wi.y);
+    if (tt % 2) {
+      if (edge)
+        momxy_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
+      else
+        momxy_xy_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane);
+    } else {
+      if (edge)
+        momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
+      else
+        momxy_yx_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane);
+    }
   }
   mq.finish();
 }
