@@ -100,10 +100,11 @@ FlowHipCtx* flowhip_ctx(Mesh* mesh) {
    * (the same bits either way) */
   const char* un = getenv("FLOWHIP_UNIFORM");
   h->no_uniform = (un && !strcmp(un, "0")) ? 1 : 0;
-  /* FLOWHIP_PAIRS=0 keeps one column per lane in the Lagrangian and
-   * mass/energy marches (the same fields either way) */
+  /* two columns per lane in the Lagrangian and mass/energy marches (the
+   * same fields either way): FLOWHIP_PAIRS=0 never, =1 on every mesh,
+   * unset on meshes of at least FK_PAIR_MIN_CELLS interior cells */
   const char* pr = getenv("FLOWHIP_PAIRS");
-  h->no_pairs = (pr && !strcmp(pr, "0")) ? 1 : 0;
+  h->pairs = !pr ? 1 : !strcmp(pr, "0") ? 0 : !strcmp(pr, "1") ? 2 : 1;
   HIPCHECK(hipDeviceSynchronize());
   mesh->hip = h;
   live_add(h);
@@ -210,7 +211,7 @@ FkMesh flowhip_fkmesh(const Mesh* mesh) {
     m.inv_edgedy = h->inv_edgedy;
     m.pow2 = h->pow2 && !h->no_pow2;
     m.uniform = m.pow2 && h->uniform && !h->no_uniform;
-    m.pairs = !h->no_pairs;
+    m.pairs = h->pairs;
   } else {
     m.inv_edgedx = m.inv_edgedy = NULL;
     m.pow2 = 0;
@@ -383,7 +384,7 @@ FkMesh32 flowhip_fkmesh32(Mesh* mesh) {
   f.inv_edgedy = h->geo32 + 3 * lx + 2 * ly;
   f.pow2 = h->pow2_f32 && !h->no_pow2;
   f.uniform = f.pow2 && h->uniform && !h->no_uniform;
-  f.pairs = !h->no_pairs;
+  f.pairs = h->pairs;
   return f;
 }
 

@@ -51,7 +51,7 @@ typedef struct FlowHipCtx {
   int uniform;        /* each spacing array holds one value throughout */
   int no_pow2;        /* keep the quotients anyway */
   int no_uniform;     /* keep the per-row geometry loads anyway */
-  int no_pairs;       /* one column per lane in every march */
+  int pairs;          /* FkMesh.pairs: 0 never, 1 large meshes, 2 always */
   /* FP32 copies of the geometry for the precision form: edgedx, edgedy,
    * celldx, celldy, 1/edgedx, 1/edgedy, each (L+1) floats */
   float* geo32;

@@ -103,6 +103,13 @@ hipError_t launch_march_mass_p(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
  * the cell and y-face arrays it reads or writes as column pairs start on
  * 8-byte boundaries (nx even, arrays 8-byte aligned; strips start at even
  * columns). */
+/* Whether the two-column form runs: FkMesh.pairs 2 always, 1 on meshes of
+ * at least FK_PAIR_MIN_CELLS interior cells. */
+inline bool pairs_wanted(int pairs, int nx, int ny) {
+  return pairs == 2 ||
+         (pairs == 1 && (long)(nx - 2 * kPad) * (ny - 2 * kPad) >= FK_PAIR_MIN_CELLS);
+}
+
 template <typename T>
 inline bool pairs_fit(const MGeo<T>& g, const MMassIO<T>& io) {
   const unsigned long long bits =
@@ -203,7 +210,7 @@ hipError_t launch_march_momxy_p(MGeo<T> g, MMom2IO<T> io, int tt, const double* 
 template <typename T>
 hipError_t launch_march_mass(MGeo<T> g, MMassIO<T> io, int tt, FkReduce r,
                              int chunk_rows, int pow2, int uniform, int pairs, hipStream_t s) {
-  if (pairs && pow2 && uniform && pairs_fit(g, io))
+  if (pairs_wanted(pairs, g.nx, g.ny) && pow2 && uniform && pairs_fit(g, io))
     return launch_march_mass_p<T, true, true, true>(g, io, tt, r, chunk_rows, s);
   if (pow2 && uniform) return launch_march_mass_p<T, true, true>(g, io, tt, r, chunk_rows, s);
   return pow2 ? launch_march_mass_p<T, true, false>(g, io, tt, r, chunk_rows, s)
@@ -216,7 +223,7 @@ template <typename T>
 hipError_t launch_march_lagrangian(MGeo<T> g, MLagIO<T> io, FkReduce r, double max_dt,
                                    int first_step, int chunk_rows, int pow2, int uniform,
                                    int pairs, hipStream_t s) {
-  if (pairs && pow2 && uniform && lag_pairs_fit(g, io))
+  if (pairs_wanted(pairs, g.nx, g.ny) && pow2 && uniform && lag_pairs_fit(g, io))
     return launch_march_lagrangian_p<T, true, true, true>(g, io, r, max_dt, first_step,
                                                           chunk_rows, s);
   if constexpr (sizeof(T) == 4) {

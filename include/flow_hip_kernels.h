@@ -34,12 +34,19 @@ typedef struct FkMesh {
    * throughout (the mass/energy and momentum marches then read the row
    * geometry once per work item instead of once per row), else 0 */
   int uniform;
-  /* 1 when the Lagrangian and mass/energy marches may hold two columns
-   * per lane on uniform power-of-two meshes (each does when nx is even and
-   * the cell and y-face arrays it reads and writes are 16-byte aligned),
-   * else 0 */
+  /* Whether the Lagrangian and mass/energy marches hold two columns per
+   * lane on uniform power-of-two meshes (each can when nx is even and the
+   * cell and y-face arrays it reads and writes are 16-byte aligned):
+   * 0 never, 1 on meshes of at least FK_PAIR_MIN_CELLS interior cells,
+   * 2 on every mesh */
   int pairs;
 } FkMesh;
+
+/* Interior cells from which the two-column marches are the faster form
+ * (measured on MI355X: slower at 1024^2, where the mesh does not fill the
+ * GPU and a two-column item is the longer critical path; faster from
+ * 2048^2) */
+#define FK_PAIR_MIN_CELLS (2048L * 2048L)
 
 /* Geometry of the FP32-state precision form: the same arrays rounded to
  * FP32 (the reciprocals are rounded from the FP64 reciprocals). */

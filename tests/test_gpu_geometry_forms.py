@@ -7,9 +7,9 @@ geometry once per work item. FLOWHIP_UNIFORM=0 and FLOWHIP_POW2=0 (read
 when a mesh is created) switch those forms off. Each problem here runs in
 all three forms, FP64 and FP32 state, and every array and the dt sequence
 must agree bit for bit; the FP64 default form is also checked against the
-oracle. The totals must agree to 1e-12: on uniform power-of-two meshes the
-Lagrangian and mass/energy marches hold two columns per lane, whose
-per-item partial sums round differently from the one-column form's.
+oracle. The totals must agree to 1e-12: on large uniform power-of-two
+meshes the Lagrangian and mass/energy marches hold two columns per lane,
+whose per-item partial sums round differently from the one-column form's.
 """
 import numpy as np
 import pytest

@@ -4,8 +4,11 @@ fields of the one-column marches bit for bit, FP64 and FP32 state.
 On uniform power-of-two meshes with an even padded width the march
 schedule runs the interior items of those two marches two columns per lane
 (strips of 120 and 124 columns); edge items go through the one-column form
-in sub-strips. FLOWHIP_PAIRS=0 (read when a mesh is created) keeps one
-column per lane. Each case runs both ways over both sweep orders; every
+in sub-strips. By default it does so from 2048^2 interior cells up;
+FLOWHIP_PAIRS=1 (read when a mesh is created) asks for it on every mesh,
+which this test uses so that small and rectangular meshes take the form
+too, and FLOWHIP_PAIRS=0 keeps one column per lane. Each case runs both
+ways over both sweep orders; every
 array and the dt sequence must agree bit for bit (the CFL minimum does not
 depend on the order the candidates are taken in), and the mass and energy
 totals to 1e-12: each lane's sums then take two columns a row over other
@@ -37,9 +40,7 @@ def _init(n):
 
 
 def _run(monkeypatch, pairs, n, steps, dt, fp32):
-    monkeypatch.delenv("FLOWHIP_PAIRS", raising=False)
-    if not pairs:
-        monkeypatch.setenv("FLOWHIP_PAIRS", "0")
+    monkeypatch.setenv("FLOWHIP_PAIRS", "1" if pairs else "0")
     rho, e = _init(n)
     if not fp32:
         return harness.gpu_run(None, n, steps, flow_amd.MARCH, dt=dt, init=(rho, e))
