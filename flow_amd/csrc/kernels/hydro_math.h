@@ -48,7 +48,7 @@ __device__ __forceinline__ T gquot(T x, T d, T rd) {
 
 /* arch min(a, b) = a < b ? a : b */
 template <typename T>
-__device__ __forceinline__ T lesser(T a, T b) {
+__host__ __device__ __forceinline__ T lesser(T a, T b) {
   return (a < b) ? a : b;
 }
 
@@ -149,9 +149,9 @@ __device__ __forceinline__ T shock_heating(T e, T P, T rho, T qxx, T qyy,
  * gamma (gamma-1) e, so that no square root is taken per cell (the
  * reduction takes one of the minimum). Cells with e < 0, where omp3's
  * sound speed is undefined, give NaN as there. */
-__device__ __forceinline__ double cell_timestep_sq(double e, double qxx,
-                                                   double qyy, double rho,
-                                                   double cdx, double cdy) {
+__host__ __device__ __forceinline__ double cell_timestep_sq(double e, double qxx,
+                                                            double qyy, double rho,
+                                                            double cdx, double cdy) {
   const double c2 = kGam * (kGam - 1.0) * e;
   if (!(c2 >= 0.0)) return __builtin_nan("");
   const double dx2 = (cdx * cdx * rho) / (c2 * rho + 2.0 * qxx);
@@ -159,10 +159,33 @@ __device__ __forceinline__ double cell_timestep_sq(double e, double qxx,
   return lesser(dx2, dy2);
 }
 
+/* Whether a cell's CFL candidate is certain to be defined (not NaN) and
+ * not below a running minimum whose square is lim2 (pass the minimum
+ * itself in the FP32-state form, whose candidates are squares already),
+ * so that leaving it out changes neither the minimum nor the count of
+ * undefined candidates. With c^2 = gamma (gamma-1) e >= 0, Q >= 0 and
+ * rho, cdx > 0, dt_x >= L iff cdx^2 rho >= L^2 (c^2 rho + 2 Qxx), which needs no
+ * quotient or root. The 1e-12 margin covers this test's rounding (a few
+ * ulps) and that of the candidate itself (at most 6 ulps in cell_timestep,
+ * one quotient in cell_timestep_sq), so a cell that passes has a candidate
+ * above the minimum. Negative or non-finite inputs and products outside
+ * [1e-280, 1e280] fail the test and are evaluated. */
+__host__ __device__ __forceinline__ bool cfl_above(double e, double qxx, double qyy,
+                                                   double rho, double cdx, double cdy,
+                                                   double lim2) {
+  const double c2 = kGam * (kGam - 1.0) * e;
+  const double m = lim2 * (1.0 + 1.0e-12);
+  const double rx = m * (c2 * rho + 2.0 * qxx), ry = m * (c2 * rho + 2.0 * qyy);
+  const double lx = cdx * cdx * rho, ly = cdy * cdy * rho;
+  return c2 >= 0.0 && qxx >= 0.0 && qyy >= 0.0 && cdx > 0.0 && cdy > 0.0 &&
+         rho >= 1.0e-250 && rho <= 1.0e250 && rx >= 1.0e-280 && ry >= 1.0e-280 &&
+         lx <= 1.0e280 && ly <= 1.0e280 && lx >= rx && ly >= ry;
+}
+
 /* CFL candidate of one cell, omp3/flow.c:87-94. Always FP64. */
-__device__ __forceinline__ double cell_timestep(double e, double qxx,
-                                                double qyy, double rho,
-                                                double cdx, double cdy) {
+__host__ __device__ __forceinline__ double cell_timestep(double e, double qxx,
+                                                         double qyy, double rho,
+                                                         double cdx, double cdy) {
   const double c_s = __builtin_sqrt(kGam * (kGam - 1.0) * e);
   const double dtx = cdx / __builtin_sqrt(c_s * c_s + 2.0 * qxx / rho);
   const double dty = cdy / __builtin_sqrt(c_s * c_s + 2.0 * qyy / rho);

@@ -1896,18 +1896,22 @@ __device__ __forceinline__ void lagrangian_box(const MGeo<T>& g, const MLagIO<T>
           }
         }
       }
-      /* FP64 CFL candidate (squared in the FP32-state form) */
-      const double cand =
-          std::is_same<T, float>::value
-              ? cell_timestep_sq(double(es), double(qx), double(qy_prev), double(ri),
-                                 double(cdx_q), double(cdy(i)))
-              : cell_timestep(double(es), double(qx), double(qy_prev), double(ri),
-                              double(cdx_q), double(cdy(i)));
-      if (cand == cand)
-        local_min = lesser(local_min, cand);
-      else
-        ++nan_count;
-      if (io.cfl) io.cfl[c] = cand;
+      /* FP64 CFL candidate (squared in the FP32-state form), unless it
+       * certainly cannot lower the lane's minimum (and none is dumped) */
+      constexpr bool kSq = std::is_same<T, float>::value;
+      if (io.cfl || !cfl_above(double(es), double(qx), double(qy_prev), double(ri),
+                               double(cdx_q), double(cdy(i)),
+                               kSq ? local_min : local_min * local_min)) {
+        const double cand = kSq ? cell_timestep_sq(double(es), double(qx), double(qy_prev),
+                                                   double(ri), double(cdx_q), double(cdy(i)))
+                                : cell_timestep(double(es), double(qx), double(qy_prev),
+                                                double(ri), double(cdx_q), double(cdy(i)));
+        if (cand == cand)
+          local_min = lesser(local_min, cand);
+        else
+          ++nan_count;
+        if (io.cfl) io.cfl[c] = cand;
+      }
     }
 
     /* advance the windows */
@@ -2117,18 +2121,23 @@ __device__ __forceinline__ void lagrangian_pair(const MGeo<T>& g, const MLagIO<T
       st_pair(io.rho_old, (long)i * nx + qb, ri);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        /* FP64 CFL candidate (squared in the FP32-state form) */
-        const double cand =
-            std::is_same<T, float>::value
-                ? cell_timestep_sq(double(es[c]), double(qx[c]), double(qyb[A][c]),
-                                   double(ri[c]), double(cdx_q[c]), double(cdy(i)))
-                : cell_timestep(double(es[c]), double(qx[c]), double(qyb[A][c]),
-                                double(ri[c]), double(cdx_q[c]), double(cdy(i)));
-        if (cand == cand)
-          local_min = lesser(local_min, cand);
-        else
-          ++nan_count;
-        if (io.cfl) io.cfl[(long)i * nx + qb + c] = cand;
+        /* FP64 CFL candidate (squared in the FP32-state form), unless it
+         * certainly cannot lower the lane's minimum (and none is dumped) */
+        constexpr bool kSq = std::is_same<T, float>::value;
+        if (io.cfl || !cfl_above(double(es[c]), double(qx[c]), double(qyb[A][c]),
+                                 double(ri[c]), double(cdx_q[c]), double(cdy(i)),
+                                 kSq ? local_min : local_min * local_min)) {
+          const double cand =
+              kSq ? cell_timestep_sq(double(es[c]), double(qx[c]), double(qyb[A][c]),
+                                     double(ri[c]), double(cdx_q[c]), double(cdy(i)))
+                  : cell_timestep(double(es[c]), double(qx[c]), double(qyb[A][c]),
+                                  double(ri[c]), double(cdx_q[c]), double(cdy(i)));
+          if (cand == cand)
+            local_min = lesser(local_min, cand);
+          else
+            ++nan_count;
+          if (io.cfl) io.cfl[(long)i * nx + qb + c] = cand;
+        }
       }
     }
     /* row i+4 takes row i's slot */
