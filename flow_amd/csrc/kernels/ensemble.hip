@@ -345,12 +345,14 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   const int nw = L.nwork;
   if (nw < 1 || L.ninst < 1 || !L.queue) return hipErrorInvalidValue;
   const int2* work = (const int2*)L.work;
-  hipLaunchKernelGGL((k_ens_lagrangian<T, P2>), dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw, kMarchMaxWaves<T>)),
+  hipLaunchKernelGGL((k_ens_lagrangian<T, P2>),
+                     dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw, kMarchMaxWaves<T>)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, tt == 0 ? 1 : 0, std::is_same<T, float>::value ? 1 : 0,
                      L.hist);
-  hipLaunchKernelGGL((k_ens_mass<T, P2>), dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw, kMarchMaxWaves<T>)),
+  hipLaunchKernelGGL((k_ens_mass<T, P2>),
+                     dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw, kMarchMaxWaves<T>)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, L.hist);

@@ -562,7 +562,9 @@ using ItemSum = typename std::conditional<sizeof(T) == 4, PlainSum, CSum>::type;
  * the x-updated rows r-3..r feeds the y flux at face r-1 and the second (y)
  * update of row r-2. */
 template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
-__device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, Acc& ms, Acc& es) {
+__device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                            T dt_h, const ItemBox box, int lane, Acc& ms,
+                                            Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -841,10 +843,8 @@ __device__ __forceinline__ void mass_xy_pair(const MGeo<T>& g, const MMassIO<T>&
  * columns; edge items go through the one-column form in sub-strips of
  * kMassW). */
 template <typename T, bool P2, bool UNI = false, bool PAIR = false>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGeo<T> g, MMassIO<T> io,
-                                                        const double* dtp,
-                                                        FkReduce red,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(
+    MGeo<T> g, MMassIO<T> io, const double* dtp, FkReduce red, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   /* each item's per-lane compensated sums, rounded once, go into exact
@@ -884,7 +884,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_xy(MGe
  * r+1 and the first (y) update of row r; then across the lanes the x fluxes
  * at faces [x0, x1] and the second (x) update of cells [x0, x1). */
 template <typename T, bool P2, bool EDGE = true, bool UNI = false, typename Acc = CSum>
-__device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& io, T dt, T dt_h, const ItemBox box, int lane, Acc& ms, Acc& es) {
+__device__ __forceinline__ void mass_yx_box(const MGeo<T>& g, const MMassIO<T>& io, T dt,
+                                            T dt_h, const ItemBox box, int lane, Acc& ms,
+                                            Acc& es) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -1148,10 +1150,8 @@ __device__ __forceinline__ void mass_yx_pair(const MGeo<T>& g, const MMassIO<T>&
 }
 
 template <typename T, bool P2, bool UNI = false, bool PAIR = false>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(MGeo<T> g, MMassIO<T> io,
-                                                        const double* dtp,
-                                                        FkReduce red,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_mass_yx(
+    MGeo<T> g, MMassIO<T> io, const double* dtp, FkReduce red, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   /* each item's per-lane compensated sums, rounded once, go into exact
@@ -1227,7 +1227,8 @@ __device__ __forceinline__ int mirror_lane(int lane, int q, int n) {
  * a window of u1 rows r-1 .. r+2. Ghost rows are the mirror row's values
  * (u is copied unsigned across the south/north walls). */
 template <typename T, bool P2>
-__device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
+__device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -1318,9 +1319,8 @@ __device__ __forceinline__ void momx_xy_item(const MGeo<T>& g, const MMomIO<T>& 
 }
 
 template <typename T, bool P2>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(
+    MGeo<T> g, MMomIO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   const MarchWalk mw = march_walk<false>(ml.nitems);
@@ -1336,7 +1336,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_xy(MGe
  * from a window of u0 rows r-1 .. r+2; ghost faces hold the negated mirror
  * face's u1 (INVERT_X) and ghost cells the mirror cell's flux. */
 template <typename T, bool P2>
-__device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
+__device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -1414,9 +1415,8 @@ __device__ __forceinline__ void momx_yx_item(const MGeo<T>& g, const MMomIO<T>& 
 }
 
 template <typename T, bool P2>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(
+    MGeo<T> g, MMomIO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   const MarchWalk mw = march_walk<false>(ml.nitems);
@@ -1433,7 +1433,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momx_yx(MGe
  * of v1 rows g-1 .. g+2; a ghost cell row's flux is its mirror row's. io.u
  * is u after the x sub-updates. */
 template <typename T, bool P2>
-__device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
+__device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -1558,9 +1559,8 @@ __device__ __forceinline__ void momy_xy_item(const MGeo<T>& g, const MMomIO<T>& 
 }
 
 template <typename T, bool P2>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(
+    MGeo<T> g, MMomIO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   const MarchWalk mw = march_walk<false>(ml.nitems);
@@ -1576,7 +1576,8 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_xy(MGe
  * row (ghost columns: the mirror column, unsigned) and the corner fluxes
  * across the lanes. io.u is u after the x sub-updates. */
 template <typename T, bool P2>
-__device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt, T dt_h, const MarchList& ml, int item, int lane) {
+__device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& io, T dt,
+                                             T dt_h, const MarchList& ml, int item, int lane) {
   const int nx = g.nx, ny = g.ny, sx = nx + 1;
   (void)dt;
   (void)sx;
@@ -1663,9 +1664,8 @@ __device__ __forceinline__ void momy_yx_item(const MGeo<T>& g, const MMomIO<T>& 
 }
 
 template <typename T, bool P2>
-__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(MGeo<T> g, MMomIO<T> io,
-                                                        const double* dtp,
-                                                        MarchList ml) {
+__global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_march_momy_yx(
+    MGeo<T> g, MMomIO<T> io, const double* dtp, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   const MarchWalk mw = march_walk<false>(ml.nitems);
@@ -2168,10 +2168,8 @@ constexpr int kLagMinWaves = (PAIR && sizeof(T) == 4) ? kLagPairWaves<T> : 1;
 
 template <typename T, bool P2, bool UNI = false, bool PAIR = false>
 __global__ __launch_bounds__(kMNT) __attribute__((amdgpu_waves_per_eu(kLagMinWaves<T, PAIR>)))
-void k_march_lagrangian(MGeo<T> g, MLagIO<T> io,
-                                                           const double* dtp,
-                                                           FkReduce red, double max_dt,
-                                                           int first_step, MarchList ml) {
+void k_march_lagrangian(MGeo<T> g, MLagIO<T> io, const double* dtp, FkReduce red,
+                        double max_dt, int first_step, MarchList ml) {
   const T dt = T(dtp[0]), dt_h = T(dtp[1]);
   const int lane = threadIdx.x % kWave;
   constexpr bool kSq = std::is_same<T, float>::value;
