@@ -41,6 +41,7 @@ struct FlowHipEnsemble {
   int nwork;
   int chunk_rows, order;
   int pow2; /* every instance's spacings are powers of two */
+  int uniform; /* spacings read once per work item */
   int fused_momentum;
   unsigned int* d_queue; /* work-queue counters of the ensemble kernels */
   FkEnsPartials part;
@@ -145,6 +146,7 @@ static FkEnsLaunch ens_launch(const FlowHipEnsemble* E) {
   L.nwork = E->nwork;
   L.chunk_rows = E->chunk_rows;
   L.pow2 = E->pow2;
+  L.uniform = E->uniform;
   L.fused_momentum = E->fused_momentum;
   L.queue = E->d_queue;
   L.part = E->part;
@@ -262,6 +264,12 @@ FlowHipEnsemble* flowhip_ensemble_new(int ninst, const int* global_nx,
   {
     const char* p2 = getenv("FLOWHIP_POW2");
     if (p2 && !strcmp(p2, "0")) E->pow2 = 0;
+  }
+  /* every spacing array of an instance holds width / global_nx (or height /
+   * global_ny) throughout; FLOWHIP_UNIFORM=0 keeps the per-row loads */
+  {
+    const char* un = getenv("FLOWHIP_UNIFORM");
+    E->uniform = (un && !strcmp(un, "0")) ? 0 : 1;
   }
   HIPCHECK(hipFree(dtmp));
   free(hbuf);

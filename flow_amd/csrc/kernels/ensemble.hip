@@ -96,7 +96,9 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     int nan_count = 0;
     const MGeo<T> g = ens_geo<T>(in);
     const MarchList ml = ens_list(in, H);
-    /* interior items take the straight-line form, as on a single mesh */
+    /* interior items take the straight-line form, as on a single mesh (the
+     * per-row geometry loads stay: the uniform form measured 1% slower here
+     * in FP32) */
     if (!item_edge(ml, g.nx, g.ny, wi.y))
       lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane, local_min, nan_count);
     else
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
   mq.finish();
 }
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     unsigned int* __restrict__ queue,
@@ -131,14 +133,14 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_mass(
     const bool edge = item_edge(ml, g.nx, g.ny, wi.y);
     if (tt % 2 == 0) {
       if (edge)
-        mass_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+        mass_xy_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
       else
-        mass_xy_item<T, P2, false>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+        mass_xy_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
     } else {
       if (edge)
-        mass_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+        mass_yx_item<T, P2, true, UNI>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
       else
-        mass_yx_item<T, P2, false>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
+        mass_yx_item<T, P2, false, UNI>(g, io, dt, dt_h, ml, wi.y, lane, ms, es);
     }
     ms = wave_csum(ms);
     es = wave_csum(es);
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum(
 }
 
 /* Both momentum directions in one march (momxy_*_item). */
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     unsigned int* __restrict__ queue,
@@ -213,14 +215,14 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_momentum_xy(
     const bool edge = item_edge(ml, g.nx, g.ny, wi.y);
     if (tt % 2) {
       if (edge)
-        momxy_xy_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
+        momxy_xy_item<T, P2, -1, true, UNI>(g, io, dt, dt_h, ml, wi.y, lane);
       else
-        momxy_xy_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane);
+        momxy_xy_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, wi.y, lane);
     } else {
       if (edge)
-        momxy_yx_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane);
+        momxy_yx_item<T, P2, -1, true, UNI>(g, io, dt, dt_h, ml, wi.y, lane);
       else
-        momxy_yx_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane);
+        momxy_yx_item<T, P2, 0, false, UNI>(g, io, dt, dt_h, ml, wi.y, lane);
     }
   }
   mq.finish();
@@ -339,7 +341,7 @@ __global__ void k_ens_set_counter(int* counter, int v) {
 
 /* ---- launch helpers ---- */
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   static int pc_lag = 0, pc_mass = 0, pc_mom = 0;
   const int nw = L.nwork;
@@ -351,15 +353,16 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, tt == 0 ? 1 : 0, std::is_same<T, float>::value ? 1 : 0,
                      L.hist);
-  hipLaunchKernelGGL((k_ens_mass<T, P2>),
-                     dim3(march_grid(k_ens_mass<T, P2>, pc_mass, nw, kMarchMaxWaves<T>)),
+  hipLaunchKernelGGL((k_ens_mass<T, P2, UNI>),
+                     dim3(march_grid(k_ens_mass<T, P2, UNI>, pc_mass, nw, kMarchMaxWaves<T>)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, tt, L.part);
   hipLaunchKernelGGL(k_ens_sums, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, L.hist);
   if (L.fused_momentum) {
     static int pc_momxy = 0;
-    hipLaunchKernelGGL((k_ens_momentum_xy<T, P2>),
-                       dim3(march_grid(k_ens_momentum_xy<T, P2>, pc_momxy, nw, kMarchMaxWaves<T>)),
+    hipLaunchKernelGGL((k_ens_momentum_xy<T, P2, UNI>),
+                       dim3(march_grid(k_ens_momentum_xy<T, P2, UNI>, pc_momxy, nw,
+                                       kMarchMaxWaves<T>)),
                        dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, tt);
   } else {
     const int gm = march_grid(k_ens_momentum<T, P2>, pc_mom, nw);
@@ -374,7 +377,12 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
 
 template <typename T>
 hipError_t ens_step(const FkEnsLaunch& L, int tt, hipStream_t s) {
-  return L.pow2 ? ens_step_p<T, true>(L, tt, s) : ens_step_p<T, false>(L, tt, s);
+  /* every instance's spacings are uniform (width / n): the mass/energy and
+   * momentum marches read their row geometry once per item unless
+   * FLOWHIP_UNIFORM=0 (the same bits either way) */
+  if (L.uniform)
+    return L.pow2 ? ens_step_p<T, true, true>(L, tt, s) : ens_step_p<T, false, true>(L, tt, s);
+  return L.pow2 ? ens_step_p<T, true, false>(L, tt, s) : ens_step_p<T, false, false>(L, tt, s);
 }
 
 template <typename T>

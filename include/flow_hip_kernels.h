@@ -316,6 +316,8 @@ typedef struct FkEnsLaunch {
   const int* work;         /* device, nwork (instance, item) pairs */
   int ninst, nwork, chunk_rows;
   int pow2;                /* every instance's spacings are powers of two */
+  int uniform;             /* read each instance's spacings once per item
+                              (they hold one value throughout) */
   int fused_momentum;      /* both momentum directions in one march */
   unsigned int* queue;     /* FK_MARCH_QUEUE_WORDS work-queue words, zero
                               between launches (each kernel resets them) */
