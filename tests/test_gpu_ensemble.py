@@ -109,3 +109,48 @@ def test_ensemble_instance_matches_oracle():
     for name, (rel, ndiff) in cmp.items():
         assert ndiff == 0, (name, rel, ndiff)
     assert np.array_equal(batch[k]["dt"], ref["dt"])
+
+
+POW2_SIZES = [8, 32, 64, 128]
+
+
+def _ensemble_of(sizes, fp32):
+    import flow_amd
+    ens = flow_amd.Ensemble(sizes, dt=DT, fp32=fp32)
+    for k, n in enumerate(sizes):
+        ens.set_state(k, *problems.ensemble_instance(n, KINDS[k], 0.3 + 0.07 * k,
+                                                     10.0 * (k + 1)))
+    dts, mass, energy, _ = ens.run(STEPS)
+    out = [dict(arrays=ens.get_all(k), dt=dts[k], mass=mass[k], energy=energy[k])
+           for k in range(len(sizes))]
+    ens.close()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("sizes", [SIZES, POW2_SIZES], ids=["uneven", "pow2"])
+def test_ensemble_geometry_forms_agree(monkeypatch, sizes, fp32):
+    """The ensemble's geometry forms give the same bits: row geometry read
+    once per item (the default; every instance has one spacing per axis),
+    per row (FLOWHIP_UNIFORM=0), and, on power-of-two spacings, IEEE
+    quotients in place of exact reciprocal products (FLOWHIP_POW2=0)."""
+    envs = [{}, {"FLOWHIP_UNIFORM": "0"}]
+    if sizes is POW2_SIZES:
+        envs += [{"FLOWHIP_POW2": "0"}, {"FLOWHIP_POW2": "0", "FLOWHIP_UNIFORM": "0"}]
+    runs = []
+    for env in envs:
+        print(f"case ensemble geometry sizes={sizes} fp32={fp32} env={env}", flush=True)
+        for k in ("FLOWHIP_UNIFORM", "FLOWHIP_POW2"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        runs.append(_ensemble_of(sizes, fp32))
+    ref = runs[0]
+    for got in runs[1:]:
+        for k in range(len(sizes)):
+            for name in STATE:
+                assert np.array_equal(got[k]["arrays"][name], ref[k]["arrays"][name]), (k, name)
+            assert np.array_equal(got[k]["dt"], ref[k]["dt"]), k
+            np.testing.assert_allclose(got[k]["mass"], ref[k]["mass"], rtol=1e-14, atol=0)
+            np.testing.assert_allclose(got[k]["energy"], ref[k]["energy"], rtol=1e-14, atol=0)
