@@ -599,10 +599,11 @@ __device__ __forceinline__ void mass_xy_box(const MGeo<T>& g, const MMassIO<T>& 
   const int rb = south ? kPad : y0 - 2;
 #pragma unroll
   for (int k = 0; k < 4; ++k) wr[k] = we[k] = T(0);
-  vy[0] = io.v0[(rb - 2) * nx + qc];
-  vy[1] = io.v0[(rb - 1) * nx + qc];
-  r0[0] = io.rho0[(rb - 2) * nx + qc];
-  r0[1] = io.rho0[(rb - 1) * nx + qc];
+  /* rows rb-2 and rb-1 feed only fluxes and updates above y0, which are not
+   * computed, except v0 of row y0-1 at the south wall (rb = y0): the flux
+   * at face y0 takes it */
+  vy[0] = r0[0] = r0[1] = T(0);
+  vy[1] = south ? io.v0[(rb - 1) * nx + qc] : T(0);
   /* row rb's inputs; later rows are loaded one step ahead */
   T a_r = io.rho0[rb * nx + qc], a_e = io.e1[rb * nx + qc];
   T a_u = io.u0[rb * sx + qf], a_v = io.v0[rb * nx + qc];
