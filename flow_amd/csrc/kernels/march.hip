@@ -10,9 +10,10 @@ namespace flowhip {
  * units of H2 rows, chunks of k units, and nsmall one-unit chunks at the end
  * of each of the 8 XCD row bands (MarchList). Measured on MI355X at 8192^2,
  * uniform chunks (k = 1, nsmall = 0) of 24 / 20 / 16 rows (Lagrangian /
- * mass-energy / momentum; 24 / 16 / 20 in FP32; with two columns per lane
+ * mass-energy / momentum; 24 / 16 / 16 in FP32; with two columns per lane
  * 12 / 12 for the FP64 Lagrangian and mass/energy marches, 10 / 10 in
- * FP32) are fastest: tapering from
+ * FP32) are fastest (FP32 momentum: 16 rows 2.8% faster than 20 once the
+ * last row of a chunk loads nothing, 12 and 14 the same as 16): tapering from
  * tall chunks (48-80 rows) down to 12-24-row ones cut the re-read prologue
  * rows but made the Lagrangian and mass/energy marches 5-10% slower. An
  * explicit chunk_rows > 0 gives uniform chunks of that height. Every
@@ -36,7 +37,7 @@ struct Taper {
 #define FLOWHIP_TAPER_MASS32 16, 1, 0
 #endif
 #ifndef FLOWHIP_TAPER_MOM32
-#define FLOWHIP_TAPER_MOM32 20, 1, 0
+#define FLOWHIP_TAPER_MOM32 16, 1, 0
 #endif
 /* the FP32 mass/energy march with two columns per lane (three waves per
  * SIMD): 10 rows (measured 8 to 32; 14 rows +0.3%, 20 +3%) */

@@ -35,7 +35,7 @@ int flowhip_get_schedule(Mesh* mesh);
 /* Rows per chunk of the row-marching kernels: 4..120 for all of them, or 0
  * (the default) for each kernel's own height, measured fastest on MI355X at
  * 8192^2: 24 / 20 / 16 rows for the Lagrangian / mass-energy / momentum
- * marches in FP64, 24 / 16 / 20 in the FP32-state form; when the
+ * marches in FP64, 24 / 16 / 16 in the FP32-state form; when the
  * Lagrangian and mass/energy marches hold two columns per lane (uniform
  * power-of-two meshes), 12 / 12 rows in FP64 and 10 / 10 in FP32. Fields
  * and dt are bitwise the same for every height. */
