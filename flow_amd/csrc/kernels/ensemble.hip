@@ -76,7 +76,7 @@ __device__ __forceinline__ int wave_sum_int(int x) {
 
 /* ---- the four stages ---- */
 
-template <typename T, bool P2>
+template <typename T, bool P2, bool UNI>
 __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     const FkEnsInst* __restrict__ insts, const int2* __restrict__ work, int nwork,
     unsigned int* __restrict__ queue,
@@ -96,13 +96,16 @@ __global__ __launch_bounds__(kMNT, FLOWHIP_MARCH_WAVES) void k_ens_lagrangian(
     int nan_count = 0;
     const MGeo<T> g = ens_geo<T>(in);
     const MarchList ml = ens_list(in, H);
-    /* interior items take the straight-line form, as on a single mesh (the
-     * per-row geometry loads stay: the uniform form measured 1% slower here
-     * in FP32) */
+    /* interior items take the straight-line form, as on a single mesh; the
+     * uniform-geometry form in FP64 only (0.3% faster there, 1% slower in
+     * FP32, where the per-row reciprocals are cheap) */
+    constexpr bool kUni = UNI && sizeof(T) == 8;
     if (!item_edge(ml, g.nx, g.ny, wi.y))
-      lagrangian_item<T, P2, 0, false>(g, io, dt, dt_h, ml, wi.y, lane, local_min, nan_count);
+      lagrangian_item<T, P2, 0, false, kUni>(g, io, dt, dt_h, ml, wi.y, lane, local_min,
+                                             nan_count);
     else
-      lagrangian_item<T, P2>(g, io, dt, dt_h, ml, wi.y, lane, local_min, nan_count);
+      lagrangian_item<T, P2, -1, true, kUni>(g, io, dt, dt_h, ml, wi.y, lane, local_min,
+                                             nan_count);
     local_min = wave_min(local_min);
     nan_count = wave_sum_int(nan_count);
     if (lane == 0) {
@@ -347,8 +350,8 @@ hipError_t ens_step_p(const FkEnsLaunch& L, int tt, hipStream_t s) {
   const int nw = L.nwork;
   if (nw < 1 || L.ninst < 1 || !L.queue) return hipErrorInvalidValue;
   const int2* work = (const int2*)L.work;
-  hipLaunchKernelGGL((k_ens_lagrangian<T, P2>),
-                     dim3(march_grid(k_ens_lagrangian<T, P2>, pc_lag, nw, kMarchMaxWaves<T>)),
+  hipLaunchKernelGGL((k_ens_lagrangian<T, P2, UNI>),
+                     dim3(march_grid(k_ens_lagrangian<T, P2, UNI>, pc_lag, nw, kMarchMaxWaves<T>)),
                      dim3(kMNT), 0, s, L.insts, work, nw, L.queue, L.chunk_rows, L.part);
   hipLaunchKernelGGL(k_ens_dt, dim3(L.ninst), dim3(kWave), 0, s, L.insts, L.ninst,
                      L.part, tt == 0 ? 1 : 0, std::is_same<T, float>::value ? 1 : 0,
